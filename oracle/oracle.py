@@ -1,0 +1,237 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Loads ``oracle/liboracle.so`` (the plain-C FP64 restatement of the reference's CPU solvers) and
+wraps it for numpy. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg import this module, always as the checker, never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class GridDef(C.Structure):
+    """GridDefinition (Geo/SphericalGrid.cs:13-31): bounds in degrees / km and node counts."""
+
+    _fields_ = [("min_lon", C.c_double), ("max_lon", C.c_double), ("min_lat", C.c_double),
+                ("max_lat", C.c_double), ("min_depth", C.c_double), ("max_depth", C.c_double),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)]
+
+    @property
+    def count(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def as_tuple(self):
+        return (self.min_lon, self.max_lon, self.min_lat, self.max_lat, self.min_depth,
+                self.max_depth, self.nx, self.ny, self.nz)
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_G = C.POINTER(GridDef)
+
+
+def _declare(L):
+    d = C.c_double
+    L.or_to_cartesian.argtypes = [d, d, d, _dp]
+    L.or_from_cartesian.argtypes = [_dp, _dp]
+    L.or_local_frame.argtypes = [d, d, _dp, _dp, _dp]
+    L.or_grid_from_spacing.argtypes = [d, d, d, d, d, d, d, d, _G]
+    L.or_fractional.argtypes = [_G, d, d, d, _dp]
+    L.or_trilinear_weights_fractional.argtypes = [_G, d, d, d, _ip, _dp]
+    L.or_interpolate.argtypes = [_G, _dp, d, d, d]
+    L.or_interpolate.restype = d
+    L.or_min_spacing_km.argtypes = [_G]
+    L.or_min_spacing_km.restype = d
+    L.or_node_coords.argtypes = [_G, _dp, _dp, _dp]
+    L.or_surface_distance_km.argtypes = [d, d, d, d]
+    L.or_surface_distance_km.restype = d
+    L.or_metric.argtypes = [_G, _dp, _dp, C.POINTER(d)]
+    L.or_point_source.argtypes = [_G, _dp, d, d, d, d, d, _ip, _dp, C.c_int]
+    L.or_point_source.restype = C.c_int
+    L.or_relax.argtypes = [_G, _dp, C.c_int, _ip, _dp, C.c_int, _dp, _ip]
+    L.or_relax.restype = C.c_int
+    L.or_fmm.argtypes = [_G, _dp, C.c_int, _ip, _dp, _dp]
+    L.or_fmm.restype = C.c_int
+    L.or_fmm_batch.argtypes = [_G, C.c_int, C.POINTER(C.c_void_p), _ip, _ip, _dp, _dp, C.c_int]
+    L.or_fmm_batch.restype = C.c_int
+    if hasattr(L, "or_table_time"):
+        _declare_rays(L)
+    if hasattr(L, "or_lsqr"):
+        _declare_lsqr(L)
+
+
+def _declare_rays(L):
+    d = C.c_double
+    L.or_table_time_fractional.argtypes = [_G, _fp, d, d, d]
+    L.or_table_time_fractional.restype = d
+    L.or_table_time.argtypes = [_G, _fp, d, d, d]
+    L.or_table_time.restype = d
+    L.or_table_gradient.argtypes = [_G, _fp, d, d, d, _dp]
+    L.or_backtrack.argtypes = [_G, _fp, d, d, d, d, d, d, d, C.c_int, _dp, _dp, _dp]
+    L.or_backtrack.restype = C.c_int
+    L.or_ray_kernel.argtypes = [_G, C.c_int, _dp, _dp, _dp, C.c_int, _ip, _dp]
+    L.or_ray_kernel.restype = C.c_int
+    L.or_ray_row.argtypes = [_G, _fp, d, d, d, _G, d, d, d, d, C.c_int, _ip, _fp, _dp, _ip]
+    L.or_ray_row.restype = C.c_int
+    L.or_ray_rows_batch.argtypes = [_G, C.POINTER(C.c_void_p), _dp, _G, C.c_int, _ip, _dp, d,
+                                    C.c_int, C.c_int, _ip, _ip, _fp, _dp]
+    L.or_ray_rows_batch.restype = C.c_int
+
+
+def _declare_lsqr(L):
+    d = C.c_double
+    L.or_csr_transpose.argtypes = [C.c_int, C.c_int, _lp, _ip, _dp, _lp, _ip, _dp]
+    L.or_spmv.argtypes = [C.c_int, _lp, _ip, _dp, _dp, _dp, C.c_int]
+    L.or_dot.argtypes = [C.c_int, _dp, _dp]
+    L.or_dot.restype = d
+    L.or_norm.argtypes = [C.c_int, _dp]
+    L.or_norm.restype = d
+    L.or_lsqr.argtypes = [C.c_int, C.c_int, _lp, _ip, _dp, _lp, _ip, _dp, _dp, d, C.c_int, d, d, d,
+                          C.c_int, _dp, _dp, _dp]
+    L.or_lsqr.restype = C.c_int
+
+
+# ---------------------------------------------------------------- geometry
+def griddef(min_lon, max_lon, min_lat, max_lat, min_depth, max_depth, nx, ny, nz) -> GridDef:
+    return GridDef(min_lon, max_lon, min_lat, max_lat, min_depth, max_depth, nx, ny, nz)
+
+
+def from_spacing(min_lon, max_lon, min_lat, max_lat, min_depth, max_depth, h_km, v_km) -> GridDef:
+    g = GridDef()
+    lib().or_grid_from_spacing(min_lon, max_lon, min_lat, max_lat, min_depth, max_depth, h_km, v_km,
+                               C.byref(g))
+    return g
+
+
+def to_cartesian(lon, lat, depth):
+    out = np.zeros(3)
+    lib().or_to_cartesian(lon, lat, depth, out)
+    return out
+
+
+def from_cartesian(v):
+    out = np.zeros(3)
+    lib().or_from_cartesian(np.ascontiguousarray(v, dtype=np.float64), out)
+    return out
+
+
+def node_coords(g: GridDef):
+    lon, lat, dep = np.zeros(g.nx), np.zeros(g.ny), np.zeros(g.nz)
+    lib().or_node_coords(C.byref(g), lon, lat, dep)
+    return lon, lat, dep
+
+
+def trilinear_weights(g: GridDef, lon, lat, depth):
+    f = np.zeros(3)
+    lib().or_fractional(C.byref(g), lon, lat, depth, f)
+    nodes, w = np.zeros(8, np.int32), np.zeros(8)
+    lib().or_trilinear_weights_fractional(C.byref(g), f[0], f[1], f[2], nodes, w)
+    return nodes, w
+
+
+def interpolate(g: GridDef, field, lon, lat, depth) -> float:
+    return lib().or_interpolate(C.byref(g), np.ascontiguousarray(field, dtype=np.float64), lon, lat, depth)
+
+
+def min_spacing_km(g: GridDef) -> float:
+    return lib().or_min_spacing_km(C.byref(g))
+
+
+def surface_distance_km(lon1, lat1, lon2, lat2) -> float:
+    return lib().or_surface_distance_km(lon1, lat1, lon2, lat2)
+
+
+def cartesian_nodes(g: GridDef):
+    """(N, 3) geocentric positions of every node, via the oracle's ToCartesian."""
+    lon, lat, dep = node_coords(g)
+    out = np.zeros((g.count, 3))
+    n = 0
+    for k in range(g.nz):
+        for j in range(g.ny):
+            for i in range(g.nx):
+                out[n] = to_cartesian(lon[i], lat[j], dep[k])
+                n += 1
+    return out
+
+
+# ---------------------------------------------------------------- eikonal
+def metric(g: GridDef):
+    hlon, hlat, hr = np.zeros(g.ny * g.nz), np.zeros(g.nz), C.c_double()
+    lib().or_metric(C.byref(g), hlon, hlat, C.byref(hr))
+    return hlon, hlat, hr.value
+
+
+def point_source(g: GridDef, slowness, lon, lat, depth, origin_time=0.0, radius_cells=3.0):
+    s = np.ascontiguousarray(slowness, dtype=np.float64).ravel()
+    cap = 4096
+    while True:
+        idx, t = np.zeros(cap, np.int32), np.zeros(cap)
+        n = lib().or_point_source(C.byref(g), s, lon, lat, depth, origin_time, radius_cells, idx, t, cap)
+        if n <= cap:
+            return idx[:n].copy(), t[:n].copy()
+        cap = n
+
+
+def relax(g: GridDef, slowness, seed_idx, seed_time, max_rounds=0):
+    out = np.zeros(g.count)
+    rounds = np.zeros(2, np.int32)
+    s = np.ascontiguousarray(slowness, dtype=np.float64).ravel()
+    ok = lib().or_relax(C.byref(g), s, len(seed_idx), np.ascontiguousarray(seed_idx, np.int32),
+                        np.ascontiguousarray(seed_time, np.float64), max_rounds, out, rounds)
+    return out, bool(ok), tuple(int(r) for r in rounds)
+
+
+def fmm(g: GridDef, slowness, seed_idx, seed_time):
+    out = np.zeros(g.count)
+    s = np.ascontiguousarray(slowness, dtype=np.float64).ravel()
+    rc = lib().or_fmm(C.byref(g), s, len(seed_idx), np.ascontiguousarray(seed_idx, np.int32),
+                      np.ascontiguousarray(seed_time, np.float64), out)
+    if rc != 0:
+        raise ValueError("no seed nodes")
+    return out
+
+
+def fmm_batch(g: GridDef, slowness_list, seed_sets, threads=0):
+    """One FMM table per field, fields spread over `threads` cores (0 = all)."""
+    nf = len(seed_sets)
+    offs = np.zeros(nf + 1, np.int32)
+    for f, (idx, _) in enumerate(seed_sets):
+        offs[f + 1] = offs[f] + len(idx)
+    idx = np.ascontiguousarray(np.concatenate([s[0] for s in seed_sets]).astype(np.int32))
+    tim = np.ascontiguousarray(np.concatenate([s[1] for s in seed_sets]).astype(np.float64))
+    slows = [np.ascontiguousarray(s, dtype=np.float64).ravel() for s in slowness_list]
+    ptrs = (C.c_void_p * nf)(*[s.ctypes.data for s in slows])
+    out = np.zeros(nf * g.count)
+    rc = lib().or_fmm_batch(C.byref(g), nf, ptrs, offs, idx, tim, out, threads)
+    if rc != 0:
+        raise ValueError("a field has no seed nodes")
+    return out.reshape(nf, g.count)

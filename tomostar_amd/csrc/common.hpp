@@ -1,0 +1,126 @@
+// Shared host plumbing of libtomostar_hip: the device context, HIP error checks that become status
+// codes at the C ABI (nothing throws across it), and an owning device buffer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "tomostar_hip.h"
+
+struct ts_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    size_t global_mem = 0;
+    int cu_count = 0;
+    double eik_selftest = NAN;
+    double lsqr_selftest = NAN;
+    std::mutex gate;  // one solve at a time per handle, as the reference's lock(_gate)
+};
+
+namespace ts {
+
+// A failure inside the library; the ABI wrapper turns it into a status and a reason string.
+struct error : std::runtime_error {
+    int status;
+    error(int st, const std::string &m) : std::runtime_error(m), status(st) {}
+};
+
+[[noreturn]] inline void fail(int status, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw error(status, buf);
+}
+
+#define TS_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            ::ts::fail(TS_EDEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+inline void set_reason(char *reason, int len, const char *msg)
+{
+    if (reason && len > 0) {
+        std::strncpy(reason, msg, (size_t)len - 1);
+        reason[len - 1] = 0;
+    }
+}
+
+// Runs `body`, mapping library errors and HIP failures to status codes. Device allocation
+// failures (out of memory) decline, so the caller falls back, like the reference's buffer errors.
+template <class F>
+int guarded(char *reason, int len, F &&body)
+{
+    try {
+        set_reason(reason, len, "");
+        return body();
+    } catch (const error &e) {
+        set_reason(reason, len, e.what());
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        set_reason(reason, len, "host memory exhausted");
+        return TS_DECLINED;
+    } catch (const std::exception &e) {
+        set_reason(reason, len, e.what());
+        return TS_EDEVICE;
+    }
+}
+
+// Owning device allocation. Out-of-memory becomes a decline (status 1) with the size in the reason.
+template <class T>
+struct dbuf {
+    T *p = nullptr;
+    size_t n = 0;
+    dbuf() = default;
+    explicit dbuf(size_t count) { alloc(count); }
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    dbuf(dbuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    dbuf &operator=(dbuf &&o) noexcept
+    {
+        if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~dbuf() { reset(); }
+    void alloc(size_t count)
+    {
+        reset();
+        if (count == 0) return;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            fail(TS_DECLINED, "device buffer of %zu MB: %s", (count * sizeof(T)) >> 20, hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void ensure(size_t count)
+    {
+        if (n < count) alloc(count);
+    }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+inline bool cancelled(const volatile int32_t *cancel) { return cancel && *cancel != 0; }
+
+}  // namespace ts

@@ -1,0 +1,921 @@
+// The batched 3-D spherical eikonal on gfx950.
+//
+// Scheme (the reference's, Forward/SphericalEikonal.cs:284-396 Relax and Forward/EikonalOpenCl.cs:387-623):
+// Godunov upwind update, Gauss-Seidel in the eight fast-sweeping orderings, each walked along the
+// diagonal planes i'+j'+k' = p; a first-order phase to its fixed point, then a phase with the
+// second-order promotion; a round of eight sweeps that lowers no node by more than 1e-7 s ends a
+// phase; a phase that has not settled within max(32, (Nx+Ny+Nz)/4) rounds is declined.
+//
+// Two device forms:
+//   plain (TS_EIK_PLAIN_FP64): the reference form on this GPU - FP64 times, one field per work item,
+//     one launch per diagonal plane, a per-(node,field) wake byte, batch-level convergence read back
+//     by the host after every round. It is the baseline the fast form is measured against.
+//   fast (TS_EIK_FAST_FP32 / TS_EIK_FAST_FP64): all fields of a table set in one pass. A wavefront
+//     updates one node for 64 fields, one field per lane, over times stored node-major in groups of
+//     64 fields ([group][node][64]), so every neighbour read is one contiguous 256-byte (FP32) line;
+//     the diagonal planes of all groups go out as one graph per round; each field keeps its own phase
+//     and round count (so every field follows Relax exactly), and a wave skips a node when no node it
+//     reads moved since its last update (one wake byte per node and group, raised with a ballot).
+//     FP32 times use the offset form of the update (every term of the quadratic is a cell crossing
+//     time measured from the smallest upwind neighbour), which keeps the error at the rounding of the
+//     stored time. FP64 times use Relax's arithmetic term for term and agree with it to the last bit.
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "eikonal_update.hpp"
+#include "geometry.hpp"
+
+namespace ts {
+
+constexpr double kSettleTol = 1e-7;  // SphericalEikonal.cs:332 SettleTolerance
+constexpr int kLanes = 64;
+
+inline int sweep_round_cap(const Grid &g)  // SphericalEikonal.cs:335
+{
+    int c = (g.nx + g.ny + g.nz) / 4;
+    return c > 32 ? c : 32;
+}
+
+// ------------------------------------------------------------------------------- host geometry
+
+// GridMetric (SphericalEikonal.cs:17-29)
+void grid_metric(const Grid &g, double *hlon, double *hlat, double *hr)
+{
+    for (int k = 0; k < g.nz; k++) {
+        double r = kEarthRadiusKm - g.depth(k);
+        hlat[k] = r * g.dlat_rad;
+        for (int j = 0; j < g.ny; j++) {
+            double v = r * cos(g.lat(j) * kDeg2Rad) * g.dlon_rad;
+            hlon[k * g.ny + j] = v > 1e-9 ? v : 1e-9;
+        }
+    }
+    *hr = g.ddepth;
+}
+
+static double interpolate(const Grid &g, const double *field, double lon, double lat, double depth)
+{
+    double fx, fy, fz, w[8];
+    int nodes[8];
+    fractional(g, lon, lat, depth, fx, fy, fz);
+    trilinear_weights(g, fx, fy, fz, nodes, w);
+    double sum = 0;
+    for (int n = 0; n < 8; n++) sum += field[nodes[n]] * w[n];
+    return sum;
+}
+
+// SphericalEikonal.PointSource (SphericalEikonal.cs:68-100)
+int point_source(const Grid &g, const double *slowness, double lon, double lat, double depth, double origin,
+                 double radius_cells, int32_t *idx_out, double *t_out, int cap)
+{
+    double dep = dclamp(depth, g.depth(0), g.depth_last);
+    double fx, fy, fz;
+    fractional(g, lon, lat, dep, fx, fy, fz);
+    fx = dclamp(fx, 0, g.nx - 1);
+    fy = dclamp(fy, 0, g.ny - 1);
+    fz = dclamp(fz, 0, g.nz - 1);
+    Vec3 src = to_cartesian(lon, lat, dep);
+    double s_src = interpolate(g, slowness, lon, lat, dep);
+    double r = kEarthRadiusKm - dep;
+    double h_lon = r * cos(lat * kDeg2Rad) * g.dlon_rad;
+    if (!(h_lon > 1e-9)) h_lon = 1e-9;
+    double h_lat = r * g.dlat_rad, h_r = g.ddepth;
+    double radius = radius_cells * std::max(h_lon, std::max(h_lat, h_r));
+    int ri = (int)ceil(radius / h_lon), rj = (int)ceil(radius / h_lat), rk = (int)ceil(radius / h_r);
+    int i0 = (int)nearbyint(fx), j0 = (int)nearbyint(fy), k0 = (int)nearbyint(fz);  // Math.Round: half to even
+    int count = 0;
+    for (int k = std::max(0, k0 - rk); k <= std::min(g.nz - 1, k0 + rk); k++)
+        for (int j = std::max(0, j0 - rj); j <= std::min(g.ny - 1, j0 + rj); j++)
+            for (int i = std::max(0, i0 - ri); i <= std::min(g.nx - 1, i0 + ri); i++) {
+                int idx = g.index(i, j, k);
+                double d = vlen(vsub(to_cartesian(g.lon(i), g.lat(j), g.depth(k)), src));
+                bool inside = fabs(i - fx) <= 1 && fabs(j - fy) <= 1 && fabs(k - fz) <= 1;
+                if (d > radius && !inside) continue;
+                if (count < cap) {
+                    idx_out[count] = idx;
+                    t_out[count] = origin + d * 0.5 * (s_src + slowness[idx]);
+                }
+                count++;
+            }
+    return count;
+}
+
+// ------------------------------------------------------------------------------- the update
+
+// ------------------------------------------------------------------------------- plain form
+
+// One diagonal plane of one sweep ordering for a batch of fields of one slowness (the reference's
+// launch shape: item (a*batch + e, c) is i' = i0 + a, k' = k0 + c of field e, node-major fields).
+__global__ __launch_bounds__(64) void eik_plain_plane(double *__restrict__ f, const double *__restrict__ slow,
+                                                      const double *__restrict__ hlon,
+                                                      const double *__restrict__ hlat, double hr,
+                                                      const uint8_t *__restrict__ seeded, int nx, int ny, int nz,
+                                                      int second, int *__restrict__ changed, int dir, int p,
+                                                      int batch, uint8_t *__restrict__ dirty)
+{
+    const int i0 = max(0, p - (ny - 1) - (nz - 1));
+    const int iw = min(nx - 1, p) - i0 + 1;
+    const int k0 = max(0, p - (nx - 1) - (ny - 1));
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const int a = t / batch;
+    if (a >= iw) return;
+    const int e = t - a * batch;
+    const int ip = i0 + a, kp = k0 + blockIdx.y, jp = p - ip - kp;
+    if (jp < 0 || jp >= ny) return;
+    const int i = (dir & 1) ? nx - 1 - ip : ip;
+    const int j = (dir & 2) ? ny - 1 - jp : jp;
+    const int k = (dir & 4) ? nz - 1 - kp : kp;
+    const int nxy = nx * ny;
+    const int node = (k * ny + j) * nx + i;
+    const size_t b = (size_t)batch;
+    const size_t q = (size_t)node * b + e;
+    if (seeded[q] || !dirty[q]) return;
+    dirty[q] = 0;
+    const double s = slow[node];
+    if (!(s > 0.0)) return;
+    const double *fe = f + e;
+    const double cur = fe[(size_t)node * b];
+    const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
+    const double h[3] = {hlon[k * ny + j], hlat[k], hr};
+    double first[3], t2[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        double mn = pos[ax] > 0 ? fe[(size_t)(node - st[ax]) * b] : INFINITY;
+        double pl = pos[ax] + 1 < n[ax] ? fe[(size_t)(node + st[ax]) * b] : INFINITY;
+        int sec;
+        if (mn <= pl) { first[ax] = mn; sec = pos[ax] > 1 ? node - 2 * st[ax] : -1; }
+        else { first[ax] = pl; sec = pos[ax] + 2 < n[ax] ? node + 2 * st[ax] : -1; }
+        t2[ax] = (second && sec >= 0 && isfinite(first[ax])) ? fe[(size_t)sec * b] : INFINITY;
+    }
+    const double cand = godunov_abs<double>(s, first, t2, h);
+    if (!(cand < cur)) return;
+    f[q] = cand;
+    if (!isfinite(cur) || cur - cand > kSettleTol) atomicOr(changed, 1);
+    uint8_t *d = dirty + e;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        if (pos[ax] > 0) d[(size_t)(node - st[ax]) * b] = 1;
+        if (pos[ax] > 1) d[(size_t)(node - 2 * st[ax]) * b] = 1;
+        if (pos[ax] + 1 < n[ax]) d[(size_t)(node + st[ax]) * b] = 1;
+        if (pos[ax] + 2 < n[ax]) d[(size_t)(node + 2 * st[ax]) * b] = 1;
+    }
+}
+
+__global__ void eik_plain_clear(double *f, uint8_t *seeded, uint8_t *dirty, size_t total)
+{
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    f[q] = INFINITY;
+    seeded[q] = 0;
+    dirty[q] = 1;
+}
+
+__global__ void eik_fill_u8(uint8_t *p, size_t total, uint8_t v)
+{
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < total) p[q] = v;
+}
+
+__global__ void eik_plain_seed(double *f, uint8_t *seeded, const int64_t *at, const double *t, int count)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    f[at[q]] = t[q];
+    seeded[at[q]] = 1;
+}
+
+// node-major batch (x*batch + e) -> field-major FP64 and FP32 tables
+__global__ void eik_plain_unpack(const double *f, int batch, int N, double *out64, float *out32)
+{
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)batch * N) return;
+    int e = (int)(q / N);
+    size_t node = q - (size_t)e * N;
+    double v = f[node * batch + e];
+    if (out64) out64[q] = v;
+    if (out32) out32[q] = isfinite(v) ? (float)v : INFINITY;
+}
+
+// ------------------------------------------------------------------------------- fast form
+
+template <typename R>
+struct FastArgs {
+    R *T;                         // [groups][N][64]
+    uint8_t *dirty;               // [groups][N]: some node this one reads moved since its last update
+    const uint64_t *seedmask;     // [groups][N]: lanes whose field holds this node fixed
+    const R *slow;                // [models][N]
+    const int32_t *model;         // [groups*64]
+    const R *hlon, *hlat;         // [ny*nz], [nz]
+    R hr;
+    const uint64_t *active;       // [groups]: lanes still relaxing
+    const uint64_t *phase1;       // [groups]: lanes in the second-order phase
+    uint64_t *changed;            // [groups]: lanes that moved a node by more than the settle tolerance
+    unsigned long long *evals;    // [1024] spread counters of executed node updates
+    const uint32_t *planes;       // nodes of the plane family, packed i' | j' << 11 | k' << 22
+    int nx, ny, nz, N;
+};
+
+template <typename R, int WPB, bool kCount>
+__global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int dir, int off, int cnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int a = blockIdx.x * WPB + (threadIdx.x >> 6);
+    const int g = blockIdx.y;
+    if (a >= cnt) return;
+    const uint64_t act = A.active[g];
+    if (act == 0) return;
+    const uint32_t pk = A.planes[off + a];
+    const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
+    const int ip = pk & 2047, jp = (pk >> 11) & 2047, kp = pk >> 22;
+    const int i = (dir & 1) ? nx - 1 - ip : ip;
+    const int j = (dir & 2) ? ny - 1 - jp : jp;
+    const int k = (dir & 4) ? nz - 1 - kp : kp;
+    const int node = (k * ny + j) * nx + i;
+    const size_t gn = (size_t)g * A.N;
+    uint8_t *flag = A.dirty + gn + node;
+    if (*flag == 0) return;  // nothing this node reads has moved: the update would return its value
+    if (lane == 0) *flag = 0;
+    const uint64_t fixed = A.seedmask[gn + node];
+    const bool on = ((act & ~fixed) >> lane) & 1;
+    if (kCount && lane == 0) atomicAdd(&A.evals[blockIdx.x & 1023], (unsigned long long)__popcll(act & ~fixed));
+    R *Tg = A.T + gn * kLanes + lane;
+    bool dropped = false, moved = false;
+    if (on) {
+        const R s = A.slow[(size_t)A.model[g * kLanes + lane] * A.N + node];
+        const bool second = (A.phase1[g] >> lane) & 1;
+        const R cur = Tg[(size_t)node * kLanes];
+        const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
+        const R h[3] = {A.hlon[k * ny + j], A.hlat[k], A.hr};
+        R first[3], t2[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            R mn = pos[ax] > 0 ? Tg[(size_t)(node - st[ax]) * kLanes] : (R)INFINITY;
+            R pl = pos[ax] + 1 < n[ax] ? Tg[(size_t)(node + st[ax]) * kLanes] : (R)INFINITY;
+            int sec;
+            if (mn <= pl) { first[ax] = mn; sec = pos[ax] > 1 ? node - 2 * st[ax] : -1; }
+            else { first[ax] = pl; sec = pos[ax] + 2 < n[ax] ? node + 2 * st[ax] : -1; }
+            t2[ax] = (second && sec >= 0 && isfinite(first[ax])) ? Tg[(size_t)sec * kLanes] : (R)INFINITY;
+        }
+        R next;
+        if constexpr (sizeof(R) == 4) next = (s > 0.0f) ? godunov_offset(s, first, t2, h) : (R)INFINITY;
+        else next = (s > 0.0) ? godunov_abs<double>(s, first, t2, h) : (R)INFINITY;
+        if (next < cur) {
+            Tg[(size_t)node * kLanes] = next;
+            dropped = true;
+            moved = !isfinite(cur) || (double)(cur - next) > kSettleTol;
+        }
+    }
+    const uint64_t woke = __ballot(dropped);
+    if (woke) {
+        // Wake the twelve nodes that read this one (one, two steps on each axis), lanes 0-11.
+        if (lane < 12) {
+            const int ax = lane >> 2, which = lane & 3;
+            const int pos = ax == 0 ? i : ax == 1 ? j : k, n = ax == 0 ? nx : ax == 1 ? ny : nz;
+            const int st = ax == 0 ? 1 : ax == 1 ? nx : nxy;
+            const int d = which == 0 ? -1 : which == 1 ? -2 : which == 2 ? 1 : 2;
+            const int np = pos + d;
+            if (np >= 0 && np < n) A.dirty[gn + node + d * st] = 1;
+        }
+    }
+    const uint64_t mv = __ballot(moved);
+    if (mv && lane == 0) {
+        uint64_t seen = __hip_atomic_load(&A.changed[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (mv & ~seen) atomicOr((unsigned long long *)&A.changed[g], (unsigned long long)mv);
+    }
+}
+
+// Per-field bookkeeping after a round (one wave per group): a field whose round moved no node by
+// more than the tolerance ends its phase (first order -> second order -> done); a field that
+// reaches the round cap in a phase is unsettled. state: 0/1 phase, 2 done, 3 unsettled, 4 unused.
+__global__ __launch_bounds__(64) void eik_fast_round_end(uint64_t *active, uint64_t *phase1, uint64_t *changed,
+                                                         int32_t *state, int32_t *rounds, int cap,
+                                                         int32_t *status, uint8_t *reset)
+{
+    const int g = blockIdx.x, lane = threadIdx.x, f = g * 64 + lane;
+    int st = state[f];
+    const bool ch = (changed[g] >> lane) & 1;
+    bool need_reset = false;
+    if (st < 2) {
+        int r = ++rounds[2 * f + st];
+        if (!ch) {
+            if (st == 0) { st = 1; need_reset = true; }
+            else st = 2;
+        } else if (r >= cap) st = 3;
+        state[f] = st;
+    }
+    const uint64_t act = __ballot(st < 2), ph = __ballot(st == 1), rs = __ballot(need_reset);
+    const uint64_t bad = __ballot(st == 3);
+    if (lane == 0) {
+        active[g] = act;
+        phase1[g] = ph;
+        changed[g] = 0;
+        reset[g] = rs != 0;
+        if (act) atomicAdd(&status[0], __popcll(act));
+        if (bad) atomicAdd(&status[1], __popcll(bad));
+    }
+}
+
+// A field entering its second-order phase needs every node re-evaluated (the update changed).
+__global__ void eik_fast_reset_dirty(uint8_t *dirty, const uint8_t *reset, int N)
+{
+    const int g = blockIdx.y;
+    if (!reset[g]) return;
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < (size_t)N) dirty[(size_t)g * N + q] = 1;
+}
+
+template <typename R>
+__global__ void eik_fast_clear(R *T, uint8_t *dirty, uint64_t *seedmask, size_t nodes_total)
+{
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nodes_total * kLanes) T[q] = (R)INFINITY;
+    if (q < nodes_total) { dirty[q] = 1; seedmask[q] = 0; }
+}
+
+template <typename R>
+__global__ void eik_fast_seed(R *T, uint64_t *seedmask, const int32_t *field, const int32_t *node,
+                              const double *time, int count, int N)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const int f = field[q], g = f / kLanes, lane = f % kLanes;
+    const size_t at = (size_t)g * N + node[q];
+    T[at * kLanes + lane] = (R)time[q];
+    atomicOr((unsigned long long *)&seedmask[at], 1ull << lane);
+}
+
+// [group][node][64] -> field-major tables (FP32, and FP64 when requested), through an LDS tile so
+// both the read and the write are coalesced.
+template <typename R>
+__global__ __launch_bounds__(256) void eik_fast_unpack(const R *T, int N, int nfields, float *out32, double *out64)
+{
+    __shared__ R tile[64][65];
+    const int g = blockIdx.y, n0 = blockIdx.x * 64, t = threadIdx.x;
+    const R *Tg = T + (size_t)g * N * kLanes;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        int nn = (t >> 6) + r * 4, node = n0 + nn;
+        tile[nn][t & 63] = node < N ? Tg[(size_t)node * kLanes + (t & 63)] : (R)INFINITY;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        int fl = (t >> 6) + r * 4, f = g * kLanes + fl, node = n0 + (t & 63);
+        if (f >= nfields || node >= N) continue;
+        R v = tile[t & 63][fl];
+        size_t at = (size_t)f * N + node;
+        if (out32) out32[at] = isfinite(v) ? (float)v : INFINITY;
+        if (out64) out64[at] = (double)v;
+    }
+}
+
+__global__ void eik_f32_to_f64(const float *in, double *out, size_t n)
+{
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = (double)in[q];
+}
+
+}  // namespace ts
+
+// ================================================================================== host side
+
+using namespace ts;
+
+struct ts_tableset {
+    ts_context *ctx = nullptr;
+    Grid grid{};
+    int nmodels = 0, nfields = 0, N = 0;
+    std::vector<int32_t> model_of_field;
+    std::vector<std::vector<int32_t>> seed_node;  // merged seeds of each field (smallest time wins)
+    std::vector<std::vector<double>> seed_time;
+    std::vector<double> hlon, hlat;
+    double hr = 0;
+    dbuf<double> slow64, hlon64, hlat64;  // [models][N]
+    dbuf<float> slow32, hlon32, hlat32;
+    dbuf<uint32_t> planes;
+    std::vector<int> plane_off;           // nx+ny+nz-1 entries
+    // results
+    dbuf<float> tab32;                    // [fields][N] stored tables
+    dbuf<double> tab64;                   // [fields][N] when the last solve ran in FP64
+    bool have64 = false, solved = false;
+    std::vector<int32_t> rounds;          // 2 per field
+    double evaluations = 0, seconds = 0, sweep_seconds = 0;
+    // fast-form working storage, kept across solves
+    dbuf<float> T32;
+    dbuf<double> T64;
+    dbuf<uint8_t> dirty, reset;
+    dbuf<uint64_t> seedmask, active, phase1, changed;
+    dbuf<int32_t> model_dev, state, rounds_dev, status;
+    dbuf<unsigned long long> evals;
+    hipGraphExec_t graph = nullptr;
+    int graph_mode = -1;
+    bool graph_counting = false;
+    std::mutex gate;
+    ~ts_tableset()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
+};
+
+namespace {
+
+// Nodes of the diagonal planes i'+j'+k' = p of the unmirrored ordering, by plane then k' then i'.
+// The same list serves every ordering: the kernel mirrors each coordinate whose bit is set in dir.
+void build_planes(ts_tableset &t)
+{
+    const Grid &g = t.grid;
+    const int P = g.nx + g.ny + g.nz - 2;
+    std::vector<int> count(P + 1, 0);
+    for (int p = 0; p < P; p++) {
+        int i0 = std::max(0, p - (g.ny - 1) - (g.nz - 1)), i1 = std::min(g.nx - 1, p);
+        int k0 = std::max(0, p - (g.nx - 1) - (g.ny - 1)), k1 = std::min(g.nz - 1, p);
+        for (int kp = k0; kp <= k1; kp++) {
+            int lo = std::max(i0, p - kp - (g.ny - 1)), hi = std::min(i1, p - kp);
+            if (hi >= lo) count[p + 1] += hi - lo + 1;
+        }
+    }
+    for (int p = 0; p < P; p++) count[p + 1] += count[p];
+    std::vector<uint32_t> list(count[P]);
+    for (int p = 0; p < P; p++) {
+        int at = count[p];
+        int i0 = std::max(0, p - (g.ny - 1) - (g.nz - 1)), i1 = std::min(g.nx - 1, p);
+        int k0 = std::max(0, p - (g.nx - 1) - (g.ny - 1)), k1 = std::min(g.nz - 1, p);
+        for (int kp = k0; kp <= k1; kp++) {
+            int lo = std::max(i0, p - kp - (g.ny - 1)), hi = std::min(i1, p - kp);
+            for (int ip = lo; ip <= hi; ip++) {
+                int jp = p - ip - kp;
+                list[at++] = (uint32_t)ip | ((uint32_t)jp << 11) | ((uint32_t)kp << 22);
+            }
+        }
+    }
+    if ((int64_t)list.size() != (int64_t)g.count()) fail(TS_EDEVICE, "plane list covers %zu of %d nodes", list.size(), g.count());
+    t.plane_off = count;
+    t.planes.alloc(list.size());
+    TS_HIP(hipMemcpy(t.planes.p, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+}
+
+inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// EikonalOpenCl.SuggestBatch (EikonalOpenCl.cs:121-132) with this device's memory.
+int suggest_batch(const ts_context *ctx, int nodes)
+{
+    const int64_t real = 8;
+    int64_t n = std::max(1, nodes);
+    int64_t per_field = n * real + 2 * n;
+    int64_t budget = (int64_t)(ctx->global_mem * 0.5) - 2 * n * real;
+    int64_t by_alloc = (int64_t)ctx->global_mem / std::max<int64_t>(1, n * real);
+    int64_t by_host = (768LL << 20) / std::max<int64_t>(1, n * 8);
+    int64_t enough = std::max<int64_t>(4, (128LL << 20) / n);
+    int64_t b = std::min(std::min(budget / std::max<int64_t>(1, per_field), by_alloc), std::min(by_host, enough));
+    return (int)std::clamp<int64_t>(b, 0, 32);
+}
+
+// ---- plain form: the reference's batches, planes and per-round host flag read
+void solve_plain(ts_tableset &t, const volatile int32_t *cancel)
+{
+    ts_context *ctx = t.ctx;
+    hipStream_t st = ctx->stream;
+    const Grid &g = t.grid;
+    const int N = t.N;
+    const int cap = sweep_round_cap(g);
+    const int planes = g.nx + g.ny + g.nz - 2;
+    int B = suggest_batch(ctx, N);
+    if (B < 1) fail(TS_DECLINED, "no room for one field of %d nodes", N);
+    t.tab64.ensure((size_t)t.nfields * N);
+    t.tab32.ensure((size_t)t.nfields * N);
+    dbuf<double> f((size_t)B * N);
+    dbuf<uint8_t> seeded((size_t)B * N), dirty((size_t)B * N);
+    dbuf<int> changed(1);
+    int *h_changed = nullptr;
+    TS_HIP(hipHostMalloc((void **)&h_changed, sizeof(int), hipHostMallocDefault));
+    struct Free { int *p; ~Free() { if (p) (void)hipHostFree(p); } } free_h{h_changed};
+    for (int m = 0; m < t.nmodels; m++) {
+        std::vector<int> fields;
+        for (int q = 0; q < t.nfields; q++)
+            if (t.model_of_field[q] == m) fields.push_back(q);
+        for (size_t b0 = 0; b0 < fields.size(); b0 += B) {
+            const int batch = (int)std::min<size_t>(B, fields.size() - b0);
+            const size_t total = (size_t)batch * N;
+            std::vector<int64_t> at;
+            std::vector<double> tm;
+            for (int e = 0; e < batch; e++) {
+                int q = fields[b0 + e];
+                for (size_t s = 0; s < t.seed_node[q].size(); s++) {
+                    at.push_back((int64_t)t.seed_node[q][s] * batch + e);
+                    tm.push_back(t.seed_time[q][s]);
+                }
+            }
+            dbuf<int64_t> d_at(at.size());
+            dbuf<double> d_tm(tm.size());
+            TS_HIP(hipMemcpyAsync(d_at.p, at.data(), at.size() * 8, hipMemcpyHostToDevice, st));
+            TS_HIP(hipMemcpyAsync(d_tm.p, tm.data(), tm.size() * 8, hipMemcpyHostToDevice, st));
+            eik_plain_clear<<<blocks_for(total, 256), 256, 0, st>>>(f.p, seeded.p, dirty.p, total);
+            eik_plain_seed<<<blocks_for(at.size(), 256), 256, 0, st>>>(f.p, seeded.p, d_at.p, d_tm.p, (int)at.size());
+            TS_HIP(hipGetLastError());
+            for (int phase = 0; phase < 2; phase++) {
+                if (phase > 0) eik_fill_u8<<<blocks_for(total, 256), 256, 0, st>>>(dirty.p, total, 1);
+                bool settled = false;
+                int round;
+                for (round = 0; round < cap && !settled; round++) {
+                    if (cancelled(cancel)) fail(TS_CANCELLED, "cancelled");
+                    TS_HIP(hipMemsetAsync(changed.p, 0, sizeof(int), st));
+                    for (int dir = 0; dir < 8; dir++)
+                        for (int p = 0; p < planes; p++) {
+                            int i0 = std::max(0, p - (g.ny - 1) - (g.nz - 1)), iw = std::min(g.nx - 1, p) - i0 + 1;
+                            int k0 = std::max(0, p - (g.nx - 1) - (g.ny - 1)), kw = std::min(g.nz - 1, p) - k0 + 1;
+                            dim3 grid(blocks_for((size_t)iw * batch, 64), kw);
+                            eik_plain_plane<<<grid, 64, 0, st>>>(f.p, t.slow64.p + (size_t)m * N, t.hlon64.p,
+                                                                 t.hlat64.p, t.hr, seeded.p, g.nx, g.ny, g.nz,
+                                                                 phase, changed.p, dir, p, batch, dirty.p);
+                        }
+                    TS_HIP(hipGetLastError());
+                    TS_HIP(hipMemcpyAsync(h_changed, changed.p, sizeof(int), hipMemcpyDeviceToHost, st));
+                    TS_HIP(hipStreamSynchronize(st));
+                    settled = *h_changed == 0;
+                }
+                for (int e = 0; e < batch; e++) t.rounds[2 * fields[b0 + e] + phase] = round;
+                if (!settled) fail(TS_DECLINED, "sweeping not settled after %d rounds", cap);
+            }
+            // field-major results: the batch's fields are not contiguous in the table set in general
+            dbuf<double> o64(total);
+            dbuf<float> o32(total);
+            eik_plain_unpack<<<blocks_for(total, 256), 256, 0, st>>>(f.p, batch, N, o64.p, o32.p);
+            for (int e = 0; e < batch; e++) {
+                int q = fields[b0 + e];
+                TS_HIP(hipMemcpyAsync(t.tab64.p + (size_t)q * N, o64.p + (size_t)e * N, (size_t)N * 8,
+                                      hipMemcpyDeviceToDevice, st));
+                TS_HIP(hipMemcpyAsync(t.tab32.p + (size_t)q * N, o32.p + (size_t)e * N, (size_t)N * 4,
+                                      hipMemcpyDeviceToDevice, st));
+            }
+            TS_HIP(hipStreamSynchronize(st));
+        }
+    }
+    t.have64 = true;
+}
+
+// ---- fast form
+template <typename R>
+FastArgs<R> fast_args(ts_tableset &t, R *T)
+{
+    FastArgs<R> A;
+    A.T = T;
+    A.dirty = t.dirty.p;
+    A.seedmask = t.seedmask.p;
+    if constexpr (sizeof(R) == 4) { A.slow = t.slow32.p; A.hlon = t.hlon32.p; A.hlat = t.hlat32.p; A.hr = (float)t.hr; }
+    else { A.slow = t.slow64.p; A.hlon = t.hlon64.p; A.hlat = t.hlat64.p; A.hr = t.hr; }
+    A.model = t.model_dev.p;
+    A.active = t.active.p;
+    A.phase1 = t.phase1.p;
+    A.changed = t.changed.p;
+    A.evals = t.evals.p;
+    A.planes = t.planes.p;
+    A.nx = t.grid.nx; A.ny = t.grid.ny; A.nz = t.grid.nz; A.N = t.N;
+    return A;
+}
+
+constexpr int kWPB = 4;
+
+template <typename R, bool kCount>
+void launch_round(ts_tableset &t, R *T, hipStream_t st)
+{
+    const int groups = (t.nfields + kLanes - 1) / kLanes;
+    FastArgs<R> A = fast_args<R>(t, T);
+    const int P = t.grid.nx + t.grid.ny + t.grid.nz - 2;
+    for (int dir = 0; dir < 8; dir++)
+        for (int p = 0; p < P; p++) {
+            int off = t.plane_off[p], cnt = t.plane_off[p + 1] - off;
+            dim3 grid(blocks_for(cnt, kWPB), groups);
+            eik_fast_plane<R, kWPB, kCount><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt);
+        }
+    eik_fast_round_end<<<groups, 64, 0, st>>>(t.active.p, t.phase1.p, t.changed.p, t.state.p, t.rounds_dev.p,
+                                              sweep_round_cap(t.grid), t.status.p, t.reset.p);
+    dim3 rg(blocks_for(t.N, 256), groups);
+    eik_fast_reset_dirty<<<rg, 256, 0, st>>>(t.dirty.p, t.reset.p, t.N);
+}
+
+template <typename R>
+void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool count)
+{
+    ts_context *ctx = t.ctx;
+    hipStream_t st = ctx->stream;
+    const int N = t.N, groups = (t.nfields + kLanes - 1) / kLanes, slots = groups * kLanes;
+    const size_t nodes_total = (size_t)groups * N;
+    dbuf<R> &Tb = [&]() -> dbuf<R> & { if constexpr (sizeof(R) == 4) return t.T32; else return t.T64; }();
+    Tb.ensure(nodes_total * kLanes);
+    t.dirty.ensure(nodes_total);
+    t.seedmask.ensure(nodes_total);
+    t.active.ensure(groups); t.phase1.ensure(groups); t.changed.ensure(groups); t.reset.ensure(groups);
+    t.model_dev.ensure(slots); t.state.ensure(slots); t.rounds_dev.ensure(2 * slots); t.status.ensure(2);
+    t.evals.ensure(1024);
+    t.tab32.ensure((size_t)t.nfields * N);
+    if (sizeof(R) == 8) t.tab64.ensure((size_t)t.nfields * N);
+
+    // per-field state: all fields start in the first-order phase
+    std::vector<int32_t> model(slots, 0), state(slots, 4);
+    std::vector<uint64_t> active(groups, 0);
+    for (int f = 0; f < t.nfields; f++) {
+        model[f] = t.model_of_field[f];
+        state[f] = 0;
+        active[f / kLanes] |= 1ull << (f % kLanes);
+    }
+    TS_HIP(hipMemcpyAsync(t.model_dev.p, model.data(), slots * 4, hipMemcpyHostToDevice, st));
+    TS_HIP(hipMemcpyAsync(t.state.p, state.data(), slots * 4, hipMemcpyHostToDevice, st));
+    TS_HIP(hipMemcpyAsync(t.active.p, active.data(), groups * 8, hipMemcpyHostToDevice, st));
+    TS_HIP(hipMemsetAsync(t.phase1.p, 0, groups * 8, st));
+    TS_HIP(hipMemsetAsync(t.changed.p, 0, groups * 8, st));
+    TS_HIP(hipMemsetAsync(t.rounds_dev.p, 0, 2 * slots * 4, st));
+    TS_HIP(hipMemsetAsync(t.evals.p, 0, 1024 * 8, st));
+    eik_fast_clear<R><<<blocks_for(nodes_total * kLanes, 256), 256, 0, st>>>(Tb.p, t.dirty.p, t.seedmask.p, nodes_total);
+    {
+        std::vector<int32_t> sf, sn;
+        std::vector<double> sv;
+        for (int f = 0; f < t.nfields; f++)
+            for (size_t q = 0; q < t.seed_node[f].size(); q++) {
+                sf.push_back(f);
+                sn.push_back(t.seed_node[f][q]);
+                sv.push_back(t.seed_time[f][q]);
+            }
+        dbuf<int32_t> df(sf.size()), dn(sn.size());
+        dbuf<double> dv(sv.size());
+        TS_HIP(hipMemcpyAsync(df.p, sf.data(), sf.size() * 4, hipMemcpyHostToDevice, st));
+        TS_HIP(hipMemcpyAsync(dn.p, sn.data(), sn.size() * 4, hipMemcpyHostToDevice, st));
+        TS_HIP(hipMemcpyAsync(dv.p, sv.data(), sv.size() * 8, hipMemcpyHostToDevice, st));
+        eik_fast_seed<R><<<blocks_for(sf.size(), 256), 256, 0, st>>>(Tb.p, t.seedmask.p, df.p, dn.p, dv.p,
+                                                                      (int)sf.size(), N);
+        TS_HIP(hipGetLastError());
+        TS_HIP(hipStreamSynchronize(st));
+    }
+    // One round as a graph (8 orderings x all planes x all groups, then the per-field bookkeeping).
+    if (!t.graph || t.graph_mode != mode || t.graph_counting != count) {
+        if (t.graph) { TS_HIP(hipGraphExecDestroy(t.graph)); t.graph = nullptr; }
+        hipGraph_t gr;
+        TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        if (count) launch_round<R, true>(t, Tb.p, st);
+        else launch_round<R, false>(t, Tb.p, st);
+        TS_HIP(hipStreamEndCapture(st, &gr));
+        hipError_t e = hipGraphInstantiate(&t.graph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        TS_HIP(e);
+        t.graph_mode = mode;
+        t.graph_counting = count;
+    }
+    int32_t *h_status = nullptr;
+    TS_HIP(hipHostMalloc((void **)&h_status, 2 * sizeof(int32_t), hipHostMallocDefault));
+    struct Free { int32_t *p; ~Free() { if (p) (void)hipHostFree(p); } } free_h{h_status};
+    const int max_rounds = 2 * sweep_round_cap(t.grid) + 2;
+    bool done = false;
+    // HIP events on the solve's stream bracket each round's graph: the sweep time of the solve
+    hipEvent_t ev0, ev1;
+    TS_HIP(hipEventCreate(&ev0));
+    TS_HIP(hipEventCreate(&ev1));
+    struct FreeEv { hipEvent_t a, b; ~FreeEv() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } free_ev{ev0, ev1};
+    t.sweep_seconds = 0;
+    for (int round = 0; round < max_rounds; round++) {
+        if (cancelled(cancel)) fail(TS_CANCELLED, "cancelled");
+        TS_HIP(hipMemsetAsync(t.status.p, 0, 2 * sizeof(int32_t), st));
+        TS_HIP(hipEventRecord(ev0, st));
+        TS_HIP(hipGraphLaunch(t.graph, st));
+        TS_HIP(hipEventRecord(ev1, st));
+        TS_HIP(hipMemcpyAsync(h_status, t.status.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        TS_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        TS_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        t.sweep_seconds += ms * 1e-3;
+        if (h_status[1] > 0) fail(TS_DECLINED, "sweeping not settled after %d rounds", sweep_round_cap(t.grid));
+        if (h_status[0] == 0) { done = true; break; }
+    }
+    if (!done) fail(TS_DECLINED, "sweeping not settled");
+    dim3 ug(blocks_for(N, 64), groups);
+    eik_fast_unpack<R><<<ug, 256, 0, st>>>(Tb.p, N, t.nfields, t.tab32.p, sizeof(R) == 8 ? t.tab64.p : nullptr);
+    TS_HIP(hipGetLastError());
+    std::vector<int32_t> rounds(2 * slots);
+    TS_HIP(hipMemcpyAsync(rounds.data(), t.rounds_dev.p, rounds.size() * 4, hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> ev(1024);
+    TS_HIP(hipMemcpyAsync(ev.data(), t.evals.p, 1024 * 8, hipMemcpyDeviceToHost, st));
+    TS_HIP(hipStreamSynchronize(st));
+    for (int f = 0; f < t.nfields; f++) { t.rounds[2 * f] = rounds[2 * f]; t.rounds[2 * f + 1] = rounds[2 * f + 1]; }
+    if (count) {
+        double e = 0;
+        for (auto v : ev) e += (double)v;
+        t.evaluations = e;
+    }
+    t.have64 = sizeof(R) == 8;
+}
+
+}  // namespace
+
+// ================================================================================== C ABI
+
+extern "C" {
+
+int ts_grid_metric(const ts_grid *grid, double *hlon, double *hlat, double *hr)
+{
+    if (!grid || !hlon || !hlat || !hr || !grid_valid(*grid)) return TS_EINVAL;
+    grid_metric(make_grid(*grid), hlon, hlat, hr);
+    return TS_OK;
+}
+
+int ts_point_source(const ts_grid *grid, const double *slowness, double lon, double lat, double depth_km,
+                    double origin_time, double radius_cells, int32_t *index_out, double *time_out, int cap)
+{
+    if (!grid || !slowness || !grid_valid(*grid) || cap < 0 || (cap > 0 && (!index_out || !time_out)))
+        return TS_EINVAL;
+    if (!std::isfinite(lon) || !std::isfinite(lat) || !std::isfinite(depth_km)) return TS_EINVAL;
+    return point_source(make_grid(*grid), slowness, lon, lat, depth_km, origin_time, radius_cells, index_out,
+                        time_out, cap);
+}
+
+int ts_eik_suggest_batch(ts_context *ctx, int nodes)
+{
+    if (!ctx || nodes <= 0) return TS_EINVAL;
+    return suggest_batch(ctx, nodes);
+}
+
+int ts_tables_create(ts_context *ctx, const ts_grid *grid, int nmodels, const double *const *slowness,
+                     int nfields, const int32_t *model_of_field, const int64_t *seed_offsets,
+                     const int32_t *seed_index, const double *seed_time, ts_tableset **out, char *reason,
+                     int reason_len)
+{
+    if (out) *out = nullptr;
+    if (!ctx || !grid || !out || nmodels < 1 || !slowness || nfields < 1 || !seed_offsets ||
+        !grid_valid(*grid)) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    return guarded(reason, reason_len, [&]() -> int {
+        std::unique_ptr<ts_tableset> t(new ts_tableset);
+        t->ctx = ctx;
+        t->grid = make_grid(*grid);
+        t->nmodels = nmodels;
+        t->nfields = nfields;
+        t->N = t->grid.count();
+        const int N = t->N;
+        t->model_of_field.assign(nfields, 0);
+        for (int f = 0; f < nfields; f++) {
+            int m = model_of_field ? model_of_field[f] : 0;
+            if (m < 0 || m >= nmodels) fail(TS_EINVAL, "field %d names slowness model %d of %d", f, m, nmodels);
+            t->model_of_field[f] = m;
+        }
+        // merge seeds: per field, smallest time per node (EikonalOpenCl.cs:150-163)
+        t->seed_node.resize(nfields);
+        t->seed_time.resize(nfields);
+        std::vector<std::pair<int32_t, double>> seeds;
+        for (int f = 0; f < nfields; f++) {
+            int64_t a = seed_offsets[f], b = seed_offsets[f + 1];
+            if (b < a) fail(TS_EINVAL, "seed offsets decrease at field %d", f);
+            if (b > a && (!seed_index || !seed_time)) fail(TS_EINVAL, "seed arrays are null");
+            seeds.clear();
+            for (int64_t q = a; q < b; q++)
+                if ((uint32_t)seed_index[q] < (uint32_t)N) seeds.emplace_back(seed_index[q], seed_time[q]);
+            if (seeds.empty()) fail(TS_DECLINED, "no seed nodes for field %d", f);
+            std::stable_sort(seeds.begin(), seeds.end(),
+                             [](const auto &x, const auto &y) { return x.first < y.first; });
+            for (size_t q = 0; q < seeds.size(); q++) {
+                if (q > 0 && seeds[q].first == t->seed_node[f].back()) {
+                    if (seeds[q].second < t->seed_time[f].back()) t->seed_time[f].back() = seeds[q].second;
+                    continue;
+                }
+                t->seed_node[f].push_back(seeds[q].first);
+                t->seed_time[f].push_back(seeds[q].second);
+            }
+        }
+        t->hlon.resize((size_t)t->grid.ny * t->grid.nz);
+        t->hlat.resize(t->grid.nz);
+        grid_metric(t->grid, t->hlon.data(), t->hlat.data(), &t->hr);
+        hipStream_t st = ctx->stream;
+        TS_HIP(hipSetDevice(ctx->device));
+        t->slow64.alloc((size_t)nmodels * N);
+        t->slow32.alloc((size_t)nmodels * N);
+        std::vector<float> s32(N);
+        for (int m = 0; m < nmodels; m++) {
+            if (!slowness[m]) fail(TS_EINVAL, "slowness model %d is null", m);
+            TS_HIP(hipMemcpyAsync(t->slow64.p + (size_t)m * N, slowness[m], (size_t)N * 8, hipMemcpyHostToDevice, st));
+            for (int q = 0; q < N; q++) s32[q] = (float)slowness[m][q];
+            TS_HIP(hipMemcpyAsync(t->slow32.p + (size_t)m * N, s32.data(), (size_t)N * 4, hipMemcpyHostToDevice, st));
+            TS_HIP(hipStreamSynchronize(st));
+        }
+        t->hlon64.alloc(t->hlon.size()); t->hlat64.alloc(t->hlat.size());
+        t->hlon32.alloc(t->hlon.size()); t->hlat32.alloc(t->hlat.size());
+        std::vector<float> hl32(t->hlon.begin(), t->hlon.end()), ht32(t->hlat.begin(), t->hlat.end());
+        TS_HIP(hipMemcpy(t->hlon64.p, t->hlon.data(), t->hlon.size() * 8, hipMemcpyHostToDevice));
+        TS_HIP(hipMemcpy(t->hlat64.p, t->hlat.data(), t->hlat.size() * 8, hipMemcpyHostToDevice));
+        TS_HIP(hipMemcpy(t->hlon32.p, hl32.data(), hl32.size() * 4, hipMemcpyHostToDevice));
+        TS_HIP(hipMemcpy(t->hlat32.p, ht32.data(), ht32.size() * 4, hipMemcpyHostToDevice));
+        build_planes(*t);
+        t->rounds.assign(2 * (size_t)nfields, 0);
+        *out = t.release();
+        return TS_OK;
+    });
+}
+
+int ts_tables_solve(ts_tableset *t, int mode, const volatile int32_t *cancel, int32_t *rounds_out, char *reason,
+                    int reason_len)
+{
+    if (!t || mode < TS_EIK_PLAIN_FP64 || mode > TS_EIK_FAST_FP64) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(t->gate);
+    return guarded(reason, reason_len, [&]() -> int {
+        TS_HIP(hipSetDevice(t->ctx->device));
+        t->solved = false;
+        auto t0 = std::chrono::steady_clock::now();
+        bool count = t->evaluations == 0;
+        if (mode == TS_EIK_PLAIN_FP64) solve_plain(*t, cancel);
+        else if (mode == TS_EIK_FAST_FP32) solve_fast<float>(*t, mode, cancel, count);
+        else solve_fast<double>(*t, mode, cancel, count);
+        t->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        t->solved = true;
+        if (rounds_out) {
+            rounds_out[0] = rounds_out[1] = 0;
+            for (int f = 0; f < t->nfields; f++) {
+                rounds_out[0] = std::max(rounds_out[0], t->rounds[2 * f]);
+                rounds_out[1] = std::max(rounds_out[1], t->rounds[2 * f + 1]);
+            }
+        }
+        return TS_OK;
+    });
+}
+
+int ts_tables_read(ts_tableset *t, int first, int count, double *out)
+{
+    if (!t || !out || first < 0 || count < 0 || first + count > t->nfields) return TS_EINVAL;
+    if (!t->solved) return TS_EINVAL;
+    std::lock_guard<std::mutex> lk(t->gate);
+    return guarded(nullptr, 0, [&]() -> int {
+        TS_HIP(hipSetDevice(t->ctx->device));
+        const size_t n = (size_t)count * t->N, at = (size_t)first * t->N;
+        if (t->have64) {
+            TS_HIP(hipMemcpy(out, t->tab64.p + at, n * 8, hipMemcpyDeviceToHost));
+        } else {
+            // widen on the device in slabs, so the host never holds a second copy
+            const size_t slab = std::min<size_t>(n, 32u << 20);
+            dbuf<double> w(slab);
+            for (size_t q = 0; q < n; q += slab) {
+                size_t len = std::min(slab, n - q);
+                eik_f32_to_f64<<<blocks_for(len, 256), 256, 0, t->ctx->stream>>>(t->tab32.p + at + q, w.p, len);
+                TS_HIP(hipMemcpyAsync(out + q, w.p, len * 8, hipMemcpyDeviceToHost, t->ctx->stream));
+                TS_HIP(hipStreamSynchronize(t->ctx->stream));
+            }
+        }
+        return TS_OK;
+    });
+}
+
+int ts_tables_read_f32(ts_tableset *t, int first, int count, float *out)
+{
+    if (!t || !out || first < 0 || count < 0 || first + count > t->nfields || !t->solved) return TS_EINVAL;
+    std::lock_guard<std::mutex> lk(t->gate);
+    return guarded(nullptr, 0, [&]() -> int {
+        TS_HIP(hipSetDevice(t->ctx->device));
+        TS_HIP(hipMemcpy(out, t->tab32.p + (size_t)first * t->N, (size_t)count * t->N * 4, hipMemcpyDeviceToHost));
+        return TS_OK;
+    });
+}
+
+int ts_tables_rounds(ts_tableset *t, int32_t *rounds)
+{
+    if (!t || !rounds) return TS_EINVAL;
+    std::copy(t->rounds.begin(), t->rounds.end(), rounds);
+    return TS_OK;
+}
+
+int ts_tables_stats(ts_tableset *t, double *evaluations, double *seconds, double *sweep_seconds)
+{
+    if (!t) return TS_EINVAL;
+    if (evaluations) *evaluations = t->evaluations;
+    if (seconds) *seconds = t->seconds;
+    if (sweep_seconds) *sweep_seconds = t->sweep_seconds;
+    return TS_OK;
+}
+
+void ts_tables_release(ts_tableset *t)
+{
+    if (!t) return;
+    (void)hipSetDevice(t->ctx->device);
+    delete t;
+}
+
+int ts_eik_solve(ts_context *ctx, const ts_grid *grid, const double *slowness, int batch,
+                 const int64_t *seed_offsets, const int32_t *seed_index, const double *seed_time, double *output,
+                 int mode, const volatile int32_t *cancel, int32_t *rounds_out, char *reason, int reason_len)
+{
+    if (!ctx || !grid || !slowness || batch < 1 || !seed_offsets || !output) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(ctx->gate);
+    ts_tableset *t = nullptr;
+    const double *models[1] = {slowness};
+    int rc = ts_tables_create(ctx, grid, 1, models, batch, nullptr, seed_offsets, seed_index, seed_time, &t, reason,
+                              reason_len);
+    if (rc != TS_OK) return rc;
+    rc = ts_tables_solve(t, mode, cancel, rounds_out, reason, reason_len);
+    if (rc == TS_OK) rc = ts_tables_read(t, 0, batch, output);
+    ts_tables_release(t);
+    return rc;
+}
+
+}  // extern "C"
