@@ -46,9 +46,16 @@ extern "C" {
 #define TS_EIK_FAST_FP64 2  /* the batched schedule with FP64 times: equal to SphericalEikonal.Relax of
                                each field to the last bit */
 
+/* LSQR modes */
+#define TS_LSQR_PLAIN_FP64 0 /* the reference's device loop (LsqrOpenCl.cs:272-533): FP64, one right-hand
+                                side at a time, host-side rotations, three blocking reads an iteration */
+#define TS_LSQR_FAST_FP32 1  /* k right-hand sides share each pass over A and A^T (SpMM); FP32 values and
+                                u/v/w, FP64 row sums, x and scalars, all on the device, graph per iteration */
+
 typedef struct ts_context ts_context;
 typedef struct ts_tableset ts_tableset;
 typedef struct ts_matrix ts_matrix;
+typedef struct ts_rays ts_rays;
 
 /* The spherical grid and its metric (SphericalEikonal.cs:15-35 GridMetric): node counts, HLon[Ny*Nz],
  * HLat[Nz], HR, plus the grid bounds the ray tracer needs (GridDefinition, SphericalGrid.cs:13-31). */
@@ -66,8 +73,10 @@ int ts_hip_create(int device, int flags, ts_context **out, char *reason, int rea
 void ts_hip_destroy(ts_context *ctx);
 /* Device name and the self-test results (EikonalOpenCl.Device, LsqrOpenCl.Device, SelfTest). */
 int ts_hip_device_info(ts_context *ctx, char *buf, int len);
-/* Self-test errors measured at create time (eikonal relative to range, LSQR relative). */
+/* Self-test errors measured at create time: eikonal (FP32 and FP64 forms, relative to the field
+ * range), LSQR (plain FP64 form, relative to max |x|) and the fast FP32 LSQR form. */
 int ts_hip_selftest(ts_context *ctx, double *eikonal_err, double *lsqr_err);
+int ts_hip_selftest_lsqr_fast(ts_context *ctx, double *err);
 
 /* ---------------------------------------------------------------- grid helpers (host side)
  * GridMetric (SphericalEikonal.cs:17-29): hlon[ny*nz], hlat[nz], *hr. */
@@ -111,6 +120,48 @@ int ts_tables_rounds(ts_tableset *t, int32_t *rounds);
  * the last solve's wall time, and the device time of its sweep graphs (HIP events on its stream). */
 int ts_tables_stats(ts_tableset *t, double *evaluations, double *seconds, double *sweep_seconds);
 void ts_tables_release(ts_tableset *t);
+
+/* ---------------------------------------------------------------- rays
+ * The device half of RayKernels.Compute (Tomography/RayKernels.cs:30-115): for every arrival r, the
+ * back-trace RayTracing.Backtrack (Forward/RayTracing.cs:73-111) from events[3r..3r+2] (lon, lat,
+ * depth) to the station of table field_of_ray[r] (field_station[3f..3f+2]) on the solved table set,
+ * TCalc and the hypocentre gradient at the event (TravelTimeTable.Time/Gradient, :116-157), and
+ * the kernel on the inversion grid (RayTracing.Kernel, :126-159): nodes ascending, lengths float32.
+ * step_km is RayKernels' step, 0.5 * min(forward, inversion MinSpacingKm) (RayKernels.cs:62). */
+int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *field_station, int nrays,
+                  const int32_t *field_of_ray, const double *events, double step_km, ts_rays **out,
+                  char *reason, int reason_len);
+/* Rows in arrival order: counts[nrays] (nodes per row, -1 = no row: stalled descent or unreached
+ * event, as RayKernels skips it), row_start[nrays+1] (CSR over all arrivals, empty rows for the
+ * skipped ones), nodes/lengths[row_start[nrays]], tcalc_grad[4*nrays] (TCalc, Ge, Gn, Gz) and
+ * path_points[nrays]. Any pointer may be NULL. */
+int ts_rays_rows(ts_rays *r, int32_t *counts, int64_t *row_start, int32_t *nodes, float *lengths,
+                 double *tcalc_grad, int32_t *path_points);
+/* Non-zeros of all rows and the wall time of the trace. */
+int ts_rays_stats(ts_rays *r, int64_t *nnz, double *seconds);
+void ts_rays_release(ts_rays *r);
+
+/* ---------------------------------------------------------------- LSQR
+ * A CSR matrix resident in HBM with its stored transpose (CsrMatrix and CsrMatrix.Transpose,
+ * Numerics/SparseMatrix.cs:24-97; int32 indices as CsrMatrix's). row_start[rows+1], column[nnz],
+ * values[nnz]. keep_fp64 != 0 keeps FP64 values for the plain form (FP32 values are always kept). */
+int ts_matrix_create(ts_context *ctx, int rows, int cols, const int32_t *row_start, const int32_t *column,
+                     const double *values, int keep_fp64, ts_matrix **out, char *reason, int reason_len);
+/* Non-zeros, and the device time of the last fast solve's iterations (HIP events on its stream). */
+int ts_matrix_stats(ts_matrix *a, int64_t *nnz, double *seconds);
+void ts_matrix_release(ts_matrix *a);
+
+/* Lsqr.Solve (Numerics/Lsqr.cs:34-152) / LsqrOpenCl.TrySolve (LsqrOpenCl.cs:244-270) for k (1..64)
+ * right-hand sides b[k*m] (RHS-major), each with its own damping damp[k]: minimises
+ * ||A x_j - b_j||^2 + damp_j^2 ||x_j||^2 with the reference's stopping rules (atol, btol, conlim).
+ * Outputs per RHS: x[k*n], iterations[k], stop[k] (0 budget exhausted, 1 btol, 2 atol, 3 conlim,
+ * 4-6 machine precision, 7 bidiagonalisation terminated, 8 empty system, 9 zero right-hand side,
+ * 10 A^T b = 0: the order of Lsqr.cs:141-147), stats[3k] (bnorm, rnorm, acond) and, if not NULL,
+ * the residual norm of every iteration rnorm_history[k*max_iterations] (onIteration, Lsqr.cs:140). */
+int ts_lsqr_solve(ts_matrix *a, int k, const double *b, const double *damp, int max_iterations, double atol,
+                  double btol, double conlim, int mode, const volatile int32_t *cancel, double *x,
+                  int32_t *iterations, int32_t *stop, double *stats, double *rnorm_history, char *reason,
+                  int reason_len);
 
 #ifdef __cplusplus
 }

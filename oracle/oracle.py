@@ -100,7 +100,7 @@ def _declare_rays(L):
     L.or_backtrack.restype = C.c_int
     L.or_ray_kernel.argtypes = [_G, C.c_int, _dp, _dp, _dp, C.c_int, _ip, _dp]
     L.or_ray_kernel.restype = C.c_int
-    L.or_ray_row.argtypes = [_G, _fp, d, d, d, _G, d, d, d, d, C.c_int, _ip, _fp, _dp, _ip]
+    L.or_ray_row.argtypes = [_G, _fp, d, d, d, _G, d, d, d, d, C.c_int, _ip, _fp, _dp, C.c_void_p]
     L.or_ray_row.restype = C.c_int
     L.or_ray_rows_batch.argtypes = [_G, C.POINTER(C.c_void_p), _dp, _G, C.c_int, _ip, _dp, d,
                                     C.c_int, C.c_int, _ip, _ip, _fp, _dp]
@@ -235,3 +235,102 @@ def fmm_batch(g: GridDef, slowness_list, seed_sets, threads=0):
     if rc != 0:
         raise ValueError("a field has no seed nodes")
     return out.reshape(nf, g.count)
+
+
+# ---------------------------------------------------------------- sparse algebra and LSQR
+LSQR_STOP = {0: "iteration budget exhausted", 1: "residual reduced to tolerance (btol)",
+             2: "normal equations satisfied (atol)", 3: "condition-number limit reached (conlim)",
+             4: "residual at machine precision", 5: "normal equations at machine precision",
+             6: "condition number at machine precision", 7: "bidiagonalisation terminated",
+             8: "empty system", 9: "zero right-hand side", 10: "A^T b = 0"}
+
+
+def csr_transpose(rows, cols, rs, ci, va):
+    rs = np.ascontiguousarray(rs, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    va = np.ascontiguousarray(va, np.float64)
+    trs = np.zeros(cols + 1, np.int64)
+    tci = np.zeros(len(ci), np.int32)
+    tva = np.zeros(len(va), np.float64)
+    lib().or_csr_transpose(rows, cols, rs, ci, va, trs, tci, tva)
+    return trs, tci, tva
+
+
+def spmv(rs, ci, va, x, threads=0):
+    rs = np.ascontiguousarray(rs, np.int64)
+    y = np.zeros(len(rs) - 1)
+    lib().or_spmv(len(rs) - 1, rs, np.ascontiguousarray(ci, np.int32), np.ascontiguousarray(va, np.float64),
+                  np.ascontiguousarray(x, np.float64), y, threads)
+    return y
+
+
+def lsqr(m, n, rs, ci, va, b, damp=0.0, max_it=500, atol=1e-8, btol=1e-8, conlim=1e8, threads=0,
+         transpose=None):
+    """Lsqr.Solve (Numerics/Lsqr.cs:34-152). Returns (x, stop code, stats dict, rnorm history)."""
+    rs = np.ascontiguousarray(rs, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    va = np.ascontiguousarray(va, np.float64)
+    trs, tci, tva = transpose if transpose is not None else csr_transpose(m, n, rs, ci, va)
+    x = np.zeros(n)
+    stats = np.zeros(4)
+    hist = np.zeros(max(1, max_it))
+    stop = lib().or_lsqr(m, n, rs, ci, va, trs, tci, tva, np.ascontiguousarray(b, np.float64), damp, max_it,
+                         atol, btol, conlim, threads, x, stats, hist)
+    it = int(stats[0])
+    return x, stop, {"iterations": it, "bnorm": stats[1], "rnorm": stats[2], "acond": stats[3]}, hist[:it].copy()
+
+
+# ---------------------------------------------------------------- tables and rays
+def table_time(g: GridDef, table, lon, lat, depth) -> float:
+    return lib().or_table_time(C.byref(g), np.ascontiguousarray(table, np.float32), lon, lat, depth)
+
+
+def table_gradient(g: GridDef, table, lon, lat, depth):
+    out = np.zeros(3)
+    lib().or_table_gradient(C.byref(g), np.ascontiguousarray(table, np.float32), lon, lat, depth, out)
+    return out
+
+
+def backtrack(g: GridDef, table, station, event, step_km, cap=200000):
+    """RayTracing.Backtrack: (lon, lat, depth) arrays of the path, or None when the descent stalls."""
+    pl, pa, pd = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+    n = lib().or_backtrack(C.byref(g), np.ascontiguousarray(table, np.float32), station[0], station[1], station[2],
+                           event[0], event[1], event[2], step_km, cap, pl, pa, pd)
+    if n == -2:
+        raise ValueError("path longer than cap")
+    return None if n < 0 else (pl[:n].copy(), pa[:n].copy(), pd[:n].copy())
+
+
+def ray_kernel(inv: GridDef, path, cap=100000):
+    lon, lat, dep = (np.ascontiguousarray(a, np.float64) for a in path)
+    nodes, lens = np.zeros(cap, np.int32), np.zeros(cap)
+    n = lib().or_ray_kernel(C.byref(inv), len(lon), lon, lat, dep, cap, nodes, lens)
+    if n < 0:
+        raise ValueError("kernel longer than cap")
+    return nodes[:n].copy(), lens[:n].copy()
+
+
+def ray_rows_batch(fwd: GridDef, tables, stations, inv: GridDef, ray_table, events, step_km, cap_per_ray=4096,
+                   threads=0):
+    """RayKernels.Compute's rows for many arrivals: list of (nodes, float32 lengths, tcalc, ge, gn, gz) or None."""
+    tabs = [np.ascontiguousarray(t, np.float32) for t in tables]
+    ptrs = (C.c_void_p * len(tabs))(*[t.ctypes.data for t in tabs])
+    st = np.ascontiguousarray(stations, np.float64).ravel()
+    ev = np.ascontiguousarray(events, np.float64).ravel()
+    rt = np.ascontiguousarray(ray_table, np.int32)
+    nr = len(rt)
+    counts = np.zeros(nr, np.int32)
+    nodes = np.zeros(nr * cap_per_ray, np.int32)
+    lens = np.zeros(nr * cap_per_ray, np.float32)
+    tg = np.zeros(4 * nr)
+    lib().or_ray_rows_batch(C.byref(fwd), ptrs, st, C.byref(inv), nr, rt, ev, step_km, cap_per_ray, threads,
+                            counts, nodes, lens, tg)
+    out = []
+    for r in range(nr):
+        c = counts[r]
+        if c < 0:
+            out.append(None)
+        else:
+            a = r * cap_per_ray
+            out.append((nodes[a:a + c].copy(), lens[a:a + c].copy(), *tg[4 * r:4 * r + 4]))
+    return out

@@ -24,6 +24,7 @@ struct ts_context {
     int cu_count = 0;
     double eik_selftest = NAN;
     double lsqr_selftest = NAN;
+    double lsqr_fast_selftest = NAN;
     std::mutex gate;  // one solve at a time per handle, as the reference's lock(_gate)
 };
 

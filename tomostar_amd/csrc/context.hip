@@ -16,6 +16,7 @@
 #include "geometry.hpp"
 
 namespace ts {
+double lsqr_selftest(ts_context *ctx, double *fast_err, char *msg, int len);
 void grid_metric(const Grid &g, double *hlon, double *hlat, double *hr);
 int point_source(const Grid &g, const double *slowness, double lon, double lat, double depth, double origin,
                  double radius_cells, int32_t *idx_out, double *t_out, int cap);
@@ -149,6 +150,11 @@ int ts_hip_create(int device, int flags, ts_context **out, char *reason, int rea
                 (void)hipStreamDestroy(ctx->stream);
                 fail(TS_DECLINED, "eikonal self-test failed (%s)", msg);
             }
+            ctx->lsqr_selftest = lsqr_selftest(ctx.get(), &ctx->lsqr_fast_selftest, msg, sizeof msg);
+            if (!(ctx->lsqr_selftest <= 1e-8) || !(ctx->lsqr_fast_selftest <= 1e-5)) {
+                (void)hipStreamDestroy(ctx->stream);
+                fail(TS_DECLINED, "LSQR self-test failed (%s)", msg);
+            }
         }
         *out = ctx.release();
         return TS_OK;
@@ -166,8 +172,15 @@ void ts_hip_destroy(ts_context *ctx)
 int ts_hip_device_info(ts_context *ctx, char *buf, int len)
 {
     if (!ctx || !buf || len <= 0) return TS_EINVAL;
-    snprintf(buf, len, "%s, %d CUs, %.1f GB; self-test eikonal %.2e, LSQR %.2e", ctx->name.c_str(), ctx->cu_count,
-             ctx->global_mem / 1e9, ctx->eik_selftest, ctx->lsqr_selftest);
+    snprintf(buf, len, "%s, %d CUs, %.1f GB; self-test eikonal %.2e, LSQR %.2e (FP32 form %.2e)", ctx->name.c_str(),
+             ctx->cu_count, ctx->global_mem / 1e9, ctx->eik_selftest, ctx->lsqr_selftest, ctx->lsqr_fast_selftest);
+    return TS_OK;
+}
+
+int ts_hip_selftest_lsqr_fast(ts_context *ctx, double *err)
+{
+    if (!ctx || !err) return TS_EINVAL;
+    *err = ctx->lsqr_fast_selftest;
     return TS_OK;
 }
 

@@ -32,6 +32,9 @@ namespace ts {
 
 constexpr double kSettleTol = 1e-7;  // SphericalEikonal.cs:332 SettleTolerance
 constexpr int kLanes = 64;
+#ifndef TS_FP32_SETTLE_ULPS
+#define TS_FP32_SETTLE_ULPS 0
+#endif
 
 inline int sweep_round_cap(const Grid &g)  // SphericalEikonal.cs:335
 {
@@ -264,7 +267,10 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int di
         if (next < cur) {
             Tg[(size_t)node * kLanes] = next;
             dropped = true;
-            moved = !isfinite(cur) || (double)(cur - next) > kSettleTol;
+            if constexpr (sizeof(R) == 4 && TS_FP32_SETTLE_ULPS > 0)
+                moved = !isfinite(cur) || (cur - next) > fmaxf((float)kSettleTol, TS_FP32_SETTLE_ULPS * 1.1920929e-7f * cur);
+            else
+                moved = !isfinite(cur) || (double)(cur - next) > kSettleTol;
         }
     }
     const uint64_t woke = __ballot(dropped);
@@ -705,6 +711,13 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
 }
 
 }  // namespace
+
+namespace ts {
+const float *tableset_tables(const ts_tableset *t) { return t->solved ? t->tab32.p : nullptr; }
+Grid tableset_grid(const ts_tableset *t) { return t->grid; }
+int tableset_fields(const ts_tableset *t) { return t->nfields; }
+ts_context *tableset_ctx(const ts_tableset *t) { return t->ctx; }
+}  // namespace ts
 
 // ================================================================================== C ABI
 

@@ -1,0 +1,525 @@
+// Curved rays on gfx950: the back-trace of every arrival on its station's travel-time table and the
+// path integral of the inversion grid's trilinear basis along it - the rows of the sensitivity matrix G.
+//
+// Reference: Forward/RayTracing.cs:73-159 (Backtrack, Kernel), Forward/TravelTimeTable.cs:116-157
+// (Time, Gradient), Tomography/RayKernels.cs:62-105 (step rule, nodes ascending, float32 lengths).
+// Positions are FP64 on FP32 tables, as the reference computes them.
+//
+// Two kernels per batch of arrivals:
+//   ray_trace: one work item per arrival walks the steepest descent (7 trilinear table reads and the
+//     spherical frame per step) and writes its path; arrivals are ordered by table and position so a
+//     wavefront's rays read the same table in the same region.
+//   ray_rows: one wavefront per arrival integrates the path. Lanes compute the pieces (midpoint,
+//     trilinear weights) in parallel; the contributions then enter an LDS hash table piece by piece,
+//     eight lanes (the eight corners of one piece, distinct nodes) at a time, so every node's length
+//     is summed in the reference's order (piece order, RayTracing.cs:150-155); the nodes are finally
+//     sorted ascending with a bitonic network and cast to float32.
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "geometry.hpp"
+
+struct ts_tableset;
+
+namespace ts {
+// accessors of the table set defined in eikonal.hip
+const float *tableset_tables(const ts_tableset *t);
+Grid tableset_grid(const ts_tableset *t);
+int tableset_fields(const ts_tableset *t);
+ts_context *tableset_ctx(const ts_tableset *t);
+}  // namespace ts
+
+using namespace ts;
+
+struct ts_rays {
+    ts_context *ctx = nullptr;
+    int nrays = 0;
+    std::vector<int32_t> counts;      // nodes per row, -1: no row (stalled or unreachable)
+    std::vector<int64_t> row_start;   // CSR over the arrivals that have rows (empty rows for the others)
+    dbuf<int32_t> nodes;              // device CSR of the rows
+    dbuf<float> lengths;
+    std::vector<double> tcalc_grad;   // 4 per arrival: TCalc, Ge, Gn, Gz
+    std::vector<int32_t> path_points; // points of each path (diagnostic)
+    double seconds = 0;
+};
+
+namespace ts {
+
+// TravelTimeTable.TimeFractional (TravelTimeTable.cs:122-136): trilinear, +inf if a corner with a
+// non-zero weight is unreached.
+__device__ inline double table_time_frac(const Grid &g, const float *__restrict__ T, double fx, double fy, double fz)
+{
+    int i0, j0, k0;
+    double tx, ty, tz;
+    trilinear_base(g, fx, fy, fz, i0, j0, k0, tx, ty, tz);
+    double t = 0;
+#pragma unroll
+    for (int dk = 0; dk < 2; dk++)
+#pragma unroll
+        for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+            for (int di = 0; di < 2; di++) {
+                const double w = (di == 0 ? 1 - tx : tx) * (dj == 0 ? 1 - ty : ty) * (dk == 0 ? 1 - tz : tz);
+                if (w == 0) continue;
+                const float v = T[g.index(i0 + di, j0 + dj, k0 + dk)];
+                if (!__builtin_isfinite(v)) return INFINITY;
+                t += v * w;
+            }
+    return t;
+}
+
+__device__ inline double table_time(const Grid &g, const float *T, double lon, double lat, double depth)
+{
+    double fx, fy, fz;
+    fractional(g, lon, lat, depth, fx, fy, fz);
+    return table_time_frac(g, T, fx, fy, fz);
+}
+
+// TravelTimeTable.Gradient (TravelTimeTable.cs:143-157), s/km in (east, north, down)
+__device__ inline void table_gradient(const Grid &g, const float *T, double lon, double lat, double depth, double &ge,
+                                      double &gn, double &gz)
+{
+    double fx, fy, fz;
+    fractional(g, lon, lat, depth, fx, fy, fz);
+    const double d = 0.5;
+    const double tx = (table_time_frac(g, T, fx + d, fy, fz) - table_time_frac(g, T, fx - d, fy, fz)) / (2 * d);
+    const double ty = (table_time_frac(g, T, fx, fy + d, fz) - table_time_frac(g, T, fx, fy - d, fz)) / (2 * d);
+    const double tz = (table_time_frac(g, T, fx, fy, fz + d) - table_time_frac(g, T, fx, fy, fz - d)) / (2 * d);
+    const double r = kEarthRadiusKm - depth;
+    ge = tx / (r * cos(lat * kDeg2Rad) * g.dlon_rad);
+    gn = ty / (r * g.dlat_rad);
+    gz = tz / g.ddepth;
+}
+
+struct RayJob {
+    int ray;          // arrival index
+    int field;        // table of the arrival
+    int64_t path_at;  // first path point slot
+    int path_cap;     // points that fit
+};
+
+// RayTracing.Backtrack (RayTracing.cs:73-111). path: 3 doubles (lon, lat, depth) per point.
+// npts[ray] = points, -1 no ray (stalled / non-finite), -2 path longer than its slot.
+__global__ __launch_bounds__(64) void ray_trace(Grid g, const float *__restrict__ tables, const double *__restrict__ stations,
+                                                const double *__restrict__ events, const RayJob *__restrict__ jobs,
+                                                int njobs, double step, double *__restrict__ path,
+                                                int32_t *__restrict__ npts, double *__restrict__ tcg)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= njobs) return;
+    const RayJob job = jobs[q];
+    const float *T = tables + (size_t)job.field * g.count();
+    const double slon = stations[3 * job.field], slat = stations[3 * job.field + 1];
+    const double sdep = dclamp(stations[3 * job.field + 2], g.depth(0), g.depth_last);
+    const double lon = events[3 * job.ray], lat = events[3 * job.ray + 1], depth = events[3 * job.ray + 2];
+    double *P = path + 3 * job.path_at;
+    int n = 0;
+    const Vec3 station = to_cartesian(slon, slat, sdep);
+    Vec3 p = to_cartesian(lon, lat, depth);
+    // TCalc and the hypocentre gradient at the event (RayKernels.cs:81-82)
+    const double tcalc = table_time(g, T, lon, lat, depth);
+    double ge, gn, gz;
+    table_gradient(g, T, lon, lat, depth, ge, gn, gz);
+    tcg[4 * job.ray] = tcalc;
+    tcg[4 * job.ray + 1] = ge;
+    tcg[4 * job.ray + 2] = gn;
+    tcg[4 * job.ray + 3] = gz;
+    int status = 0;
+    P[0] = lon; P[1] = lat; P[2] = depth;
+    n = 1;
+    if (!__builtin_isfinite(tcalc)) status = -1;
+    const int max_steps = 20 * (int)ceil(vlen(vsub(p, station)) / step + 10);
+    double last_t = tcalc;
+    int stalls = 0;
+    for (int s = 0; s < max_steps && status == 0; s++) {
+        if (vlen(vsub(p, station)) <= 1.5 * step) break;
+        const GeoPoint gp = from_cartesian(p);
+        double e1, e2, e3;
+        table_gradient(g, T, gp.lon, gp.lat, gp.depth, e1, e2, e3);
+        if (!__builtin_isfinite(e1) || !__builtin_isfinite(e2) || !__builtin_isfinite(e3)) { status = -1; break; }
+        // GeoMath.LocalFrame (GeoMath.cs:71-80)
+        const double la = gp.lat * kDeg2Rad, lo = gp.lon * kDeg2Rad;
+        const double sl = sin(la), cl = cos(la), so = sin(lo), co = cos(lo);
+        const Vec3 east{-so, co, 0}, north{-sl * co, -sl * so, cl}, up{cl * co, cl * so, sl};
+        const Vec3 grad = vadd(vadd(vscale(east, e1), vscale(north, e2)), vscale(up, -e3));
+        const double norm = vlen(grad);
+        if (norm < 1e-12) { status = -1; break; }
+        p = vsub(p, vscale(grad, step / norm));
+        GeoPoint qp = from_cartesian(p);
+        qp.lon = dclamp(unwrap_lon(qp.lon, g.centre_lon), g.min_lon, g.lon_last);
+        qp.lat = dclamp(qp.lat, g.min_lat, g.lat_last);
+        qp.depth = dclamp(qp.depth, g.min_depth, g.depth_last);
+        p = to_cartesian(qp.lon, qp.lat, qp.depth);
+        if (n + 1 >= job.path_cap) { status = -2; break; }
+        P[3 * n] = qp.lon; P[3 * n + 1] = qp.lat; P[3 * n + 2] = qp.depth;
+        n++;
+        const double t = table_time(g, T, qp.lon, qp.lat, qp.depth);
+        if (!(t < last_t)) {
+            if (++stalls > 20) { status = -1; break; }
+        } else stalls = 0;
+        last_t = fmin(last_t, t);
+    }
+    if (status == 0) {
+        P[3 * n] = slon; P[3 * n + 1] = slat; P[3 * n + 2] = sdep;
+        n++;
+    }
+    npts[job.ray] = status == 0 ? n : status;
+}
+
+constexpr int kHashSlots = 2048;      // per ray: distinct nodes it may touch (load kept under 1/2)
+constexpr int kChunkSeg = 32;         // path segments integrated per pass
+constexpr int kMaxPiecesSeg = 4;      // pieces a segment may split into within one pass buffer
+constexpr int kRowCap = 1024;         // nodes a row may hold
+
+struct RowsArgs {
+    Grid inv;
+    double h;                          // piece length: half the inversion grid's smallest spacing
+    const double *path;
+    const RayJob *jobs;
+    const int32_t *npts;
+    int njobs;
+    int32_t *row_nodes;                // [job][kRowCap]
+    float *row_len;
+    int32_t *row_count;                // per job: nodes, -1 no row, -3 too many nodes
+};
+
+__device__ inline uint32_t slot_hash(int32_t node) { return ((uint32_t)node * 2654435761u) & (kHashSlots - 1); }
+
+// RayTracing.Kernel (RayTracing.cs:126-159) for one arrival per wavefront.
+__global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
+{
+    __shared__ int32_t hkey[kHashSlots];
+    __shared__ double hval[kHashSlots];
+    __shared__ int32_t cnode[kChunkSeg * kMaxPiecesSeg * 8];
+    __shared__ double cval[kChunkSeg * kMaxPiecesSeg * 8];
+    __shared__ int32_t pstart[kChunkSeg + 1];
+    __shared__ int overflow;
+    const int lane = threadIdx.x;
+    const int q = blockIdx.x;
+    if (q >= A.njobs) return;
+    const RayJob job = A.jobs[q];
+    const int np = A.npts[job.ray];
+    if (np < 0) {
+        if (lane == 0) A.row_count[q] = -1;
+        return;
+    }
+    for (int s = lane; s < kHashSlots; s += 64) hkey[s] = -1;
+    if (lane == 0) overflow = 0;
+    __syncthreads();
+    const double *P = A.path + 3 * job.path_at;
+    const Grid &g = A.inv;
+    for (int s0 = 1; s0 < np; s0 += kChunkSeg) {
+        // pass 1: each lane one segment; its pieces' eight (node, dl*w) into the chunk buffer
+        const int s = s0 + lane;
+        int pieces = 0;
+        Vec3 a{0, 0, 0}, d{0, 0, 0};
+        double dl = 0;
+        if (lane < kChunkSeg && s < np) {
+            a = to_cartesian(P[3 * (s - 1)], P[3 * (s - 1) + 1], P[3 * (s - 1) + 2]);
+            const Vec3 b = to_cartesian(P[3 * s], P[3 * s + 1], P[3 * s + 2]);
+            d = vsub(b, a);
+            const double len = vlen(d);
+            if (len > 0) {
+                pieces = (int)ceil(len / A.h);
+                if (pieces < 1) pieces = 1;
+                dl = len / pieces;
+            }
+        }
+        // pieces of this chunk beyond the buffer are integrated in a further pass of the same chunk
+        if (lane == 0) pstart[0] = 0;
+        // exclusive scan of the piece counts over the chunk's lanes
+        int incl = pieces;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        if (lane < kChunkSeg) pstart[lane + 1] = incl;
+        __syncthreads();
+        const int total = pstart[kChunkSeg];
+        const int first = incl - pieces;
+        for (int base = 0; base < total; base += kChunkSeg * kMaxPiecesSeg) {
+            const int limit = min(total - base, kChunkSeg * kMaxPiecesSeg);
+            for (int k = 0; k < pieces; k++) {
+                const int pi = first + k - base;
+                if (pi < 0 || pi >= limit) continue;
+                const double f = (k + 0.5) / pieces;
+                const GeoPoint mid = from_cartesian(vadd(a, vscale(d, f)));
+                double fx, fy, fz;
+                fractional(g, mid.lon, mid.lat, mid.depth, fx, fy, fz);
+                int nodes[8];
+                double w[8];
+                trilinear_weights(g, fx, fy, fz, nodes, w);
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    cnode[pi * 8 + c] = w[c] == 0 ? -1 : nodes[c];
+                    cval[pi * 8 + c] = dl * w[c] * 1.0;
+                }
+            }
+            __syncthreads();
+            // pass 2: pieces in order, the eight corner lanes of one piece at a time
+            const int c = lane & 7;
+            const int sub = lane >> 3;
+            for (int pi = 0; pi < limit; pi++) {
+                if (sub == 0) {
+                    const int node = cnode[pi * 8 + c];
+                    if (node >= 0) {
+                        uint32_t slot = slot_hash(node);
+                        for (int probe = 0; probe < kHashSlots; probe++) {
+                            int k0 = hkey[slot];
+                            if (k0 == node) { hval[slot] += cval[pi * 8 + c]; break; }
+                            if (k0 == -1) {
+                                int prev = atomicCAS(&hkey[slot], -1, node);
+                                if (prev == -1) { hval[slot] = cval[pi * 8 + c]; break; }
+                                if (prev == node) { hval[slot] += cval[pi * 8 + c]; break; }
+                            }
+                            slot = (slot + 1) & (kHashSlots - 1);
+                            if (probe == kHashSlots / 2) { overflow = 1; }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    __syncthreads();
+    // the row: occupied slots, ascending by node (bitonic network over kHashSlots keys in LDS)
+    for (int size = 2; size <= kHashSlots; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = lane; i < kHashSlots; i += 64) {
+                const int jx = i ^ stride;
+                if (jx > i) {
+                    const bool asc = (i & size) == 0;
+                    // empty slots (-1) sort last
+                    const uint32_t ki = (uint32_t)hkey[i], kj = (uint32_t)hkey[jx];
+                    if ((ki > kj) == asc) {
+                        hkey[i] = (int32_t)kj; hkey[jx] = (int32_t)ki;
+                        const double t = hval[i]; hval[i] = hval[jx]; hval[jx] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    int count = 0;
+    for (int i = lane; i < kHashSlots; i += 64) count += hkey[i] != -1;
+    for (int off = 32; off > 0; off >>= 1) count += __shfl_xor(count, off);
+    if (count > kRowCap || overflow) {
+        if (lane == 0) A.row_count[q] = -3;
+        return;
+    }
+    for (int i = lane; i < count; i += 64) {
+        A.row_nodes[(size_t)q * kRowCap + i] = hkey[i];
+        A.row_len[(size_t)q * kRowCap + i] = (float)hval[i];
+    }
+    if (lane == 0) A.row_count[q] = count;
+}
+
+__global__ void ray_compact(const int32_t *row_nodes, const float *row_len, const int32_t *row_count,
+                            const int64_t *row_at, int njobs, int32_t *nodes, float *lengths)
+{
+    const int q = blockIdx.x;
+    if (q >= njobs) return;
+    const int c = row_count[q];
+    const int64_t at = row_at[q];
+    for (int i = threadIdx.x; i < c; i += blockDim.x) {
+        nodes[at + i] = row_nodes[(size_t)q * kRowCap + i];
+        lengths[at + i] = row_len[(size_t)q * kRowCap + i];
+    }
+}
+
+}  // namespace ts
+
+// ================================================================================== C ABI
+
+extern "C" {
+
+int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *field_station, int nrays,
+                  const int32_t *field_of_ray, const double *events, double step_km, ts_rays **out, char *reason,
+                  int reason_len)
+{
+    if (out) *out = nullptr;
+    if (!tables || !inversion || !field_station || nrays < 0 || (nrays > 0 && (!field_of_ray || !events)) || !out ||
+        !(step_km > 0) || !grid_valid(*inversion)) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    return guarded(reason, reason_len, [&]() -> int {
+        ts_context *ctx = tableset_ctx(tables);
+        const float *tab = tableset_tables(tables);
+        if (!tab) fail(TS_EINVAL, "the table set has not been solved");
+        TS_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const Grid fwd = tableset_grid(tables);
+        const Grid inv = make_grid(*inversion);
+        const int nf = tableset_fields(tables);
+        for (int r = 0; r < nrays; r++)
+            if (field_of_ray[r] < 0 || field_of_ray[r] >= nf) fail(TS_EINVAL, "arrival %d names table %d of %d", r, field_of_ray[r], nf);
+        std::unique_ptr<ts_rays> R(new ts_rays);
+        R->ctx = ctx;
+        R->nrays = nrays;
+        R->counts.assign(nrays, -1);
+        R->tcalc_grad.assign(4 * (size_t)nrays, 0.0);
+        R->path_points.assign(nrays, 0);
+        const double h = 0.5 * min_spacing_km(inv);
+        // arrivals ordered by table, then by event position, so a wavefront shares a table and a region
+        std::vector<int> order(nrays);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            if (field_of_ray[a] != field_of_ray[b]) return field_of_ray[a] < field_of_ray[b];
+            return std::make_pair(events[3 * a + 1], events[3 * a]) < std::make_pair(events[3 * b + 1], events[3 * b]);
+        });
+        dbuf<double> d_st(3 * (size_t)nf), d_ev(3 * (size_t)std::max(1, nrays)), d_tcg(4 * (size_t)std::max(1, nrays));
+        dbuf<int32_t> d_npts(std::max(1, nrays));
+        TS_HIP(hipMemcpyAsync(d_st.p, field_station, 3 * (size_t)nf * 8, hipMemcpyHostToDevice, st));
+        if (nrays > 0) TS_HIP(hipMemcpyAsync(d_ev.p, events, 3 * (size_t)nrays * 8, hipMemcpyHostToDevice, st));
+        std::vector<int32_t> counts_sorted;
+        std::vector<std::vector<int32_t>> nodes_parts;
+        std::vector<std::vector<float>> len_parts;
+        const int batch = 65536;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int b0 = 0; b0 < nrays; b0 += batch) {
+            int nb = std::min(batch, nrays - b0);
+            // path slots sized from the straight distance; a path that outgrows its slot is traced again
+            std::vector<RayJob> jobs(nb);
+            int64_t at = 0;
+            for (int q = 0; q < nb; q++) {
+                const int r = order[b0 + q];
+                const int f = field_of_ray[r];
+                const Vec3 a = to_cartesian(events[3 * r], events[3 * r + 1], events[3 * r + 2]);
+                const Vec3 s = to_cartesian(field_station[3 * f], field_station[3 * f + 1], field_station[3 * f + 2]);
+                const int cap = 3 * (int)ceil(vlen(vsub(a, s)) / step_km) + 64;
+                jobs[q] = RayJob{r, f, at, cap};
+                at += cap;
+            }
+            for (int attempt = 0; attempt < 2; attempt++) {
+                dbuf<RayJob> d_jobs(nb);
+                dbuf<double> d_path(3 * (size_t)at);
+                TS_HIP(hipMemcpyAsync(d_jobs.p, jobs.data(), nb * sizeof(RayJob), hipMemcpyHostToDevice, st));
+                ray_trace<<<(nb + 63) / 64, 64, 0, st>>>(fwd, tab, d_st.p, d_ev.p, d_jobs.p, nb, step_km, d_path.p,
+                                                          d_npts.p, d_tcg.p);
+                TS_HIP(hipGetLastError());
+                dbuf<int32_t> rn((size_t)nb * kRowCap), rc(nb);
+                dbuf<float> rl((size_t)nb * kRowCap);
+                RowsArgs A{inv, h, d_path.p, d_jobs.p, d_npts.p, nb, rn.p, rl.p, rc.p};
+                ray_rows<<<nb, 64, 0, st>>>(A);
+                TS_HIP(hipGetLastError());
+                std::vector<int32_t> cnt(nb), npts_all(nrays);
+                TS_HIP(hipMemcpyAsync(cnt.data(), rc.p, nb * 4, hipMemcpyDeviceToHost, st));
+                TS_HIP(hipMemcpyAsync(npts_all.data(), d_npts.p, (size_t)nrays * 4, hipMemcpyDeviceToHost, st));
+                TS_HIP(hipStreamSynchronize(st));
+                // paths that outgrew their slots: trace those again with the bound of Backtrack's loop
+                std::vector<RayJob> redo;
+                int64_t at2 = 0;
+                for (int q = 0; q < nb; q++)
+                    if (npts_all[jobs[q].ray] == -2) {
+                        RayJob j = jobs[q];
+                        const int r = j.ray, f = j.field;
+                        const Vec3 a = to_cartesian(events[3 * r], events[3 * r + 1], events[3 * r + 2]);
+                        const Vec3 s = to_cartesian(field_station[3 * f], field_station[3 * f + 1], field_station[3 * f + 2]);
+                        j.path_cap = 20 * (int)ceil(vlen(vsub(a, s)) / step_km + 10) + 4;
+                        j.path_at = at2;
+                        at2 += j.path_cap;
+                        redo.push_back(j);
+                    }
+                // keep the finished rows
+                std::vector<int64_t> row_at(nb + 1, 0);
+                for (int q = 0; q < nb; q++) row_at[q + 1] = row_at[q] + std::max(0, cnt[q]);
+                dbuf<int64_t> d_row_at(nb + 1);
+                dbuf<int32_t> cn(std::max<int64_t>(1, row_at[nb]));
+                dbuf<float> cl(std::max<int64_t>(1, row_at[nb]));
+                TS_HIP(hipMemcpyAsync(d_row_at.p, row_at.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st));
+                ray_compact<<<nb, 256, 0, st>>>(rn.p, rl.p, rc.p, d_row_at.p, nb, cn.p, cl.p);
+                std::vector<int32_t> hn(row_at[nb]);
+                std::vector<float> hl(row_at[nb]);
+                if (row_at[nb] > 0) {
+                    TS_HIP(hipMemcpyAsync(hn.data(), cn.p, row_at[nb] * 4, hipMemcpyDeviceToHost, st));
+                    TS_HIP(hipMemcpyAsync(hl.data(), cl.p, row_at[nb] * 4, hipMemcpyDeviceToHost, st));
+                }
+                TS_HIP(hipStreamSynchronize(st));
+                for (int q = 0; q < nb; q++) {
+                    const int r = jobs[q].ray;
+                    if (npts_all[r] == -2 && attempt == 0) continue;
+                    R->counts[r] = cnt[q] >= 0 ? cnt[q] : -1;
+                    R->path_points[r] = npts_all[r];
+                    if (cnt[q] == -3) fail(TS_DECLINED, "ray %d touches more than %d nodes", r, kRowCap);
+                    if (cnt[q] > 0) {
+                        nodes_parts.emplace_back(hn.begin() + row_at[q], hn.begin() + row_at[q + 1]);
+                        len_parts.emplace_back(hl.begin() + row_at[q], hl.begin() + row_at[q + 1]);
+                        counts_sorted.push_back(r);
+                    }
+                }
+                if (redo.empty()) break;
+                if (attempt == 1) fail(TS_EDEVICE, "ray path exceeded the back-trace bound");
+                jobs = redo;
+                at = at2;
+                nb = (int)redo.size();
+            }
+        }
+        std::vector<double> tcg(4 * (size_t)nrays);
+        if (nrays > 0) TS_HIP(hipMemcpy(tcg.data(), d_tcg.p, tcg.size() * 8, hipMemcpyDeviceToHost));
+        R->tcalc_grad = tcg;
+        for (int r = 0; r < nrays; r++)
+            if (R->counts[r] >= 0 && !std::isfinite(tcg[4 * r])) R->counts[r] = -1;  // RayKernels.cs:94
+        // rows in arrival order (RayKernels.cs:114)
+        std::vector<int> slot(nrays, -1);
+        for (size_t i = 0; i < counts_sorted.size(); i++) slot[counts_sorted[i]] = (int)i;
+        R->row_start.assign(nrays + 1, 0);
+        for (int r = 0; r < nrays; r++)
+            R->row_start[r + 1] = R->row_start[r] + (R->counts[r] > 0 && slot[r] >= 0 ? R->counts[r] : 0);
+        const int64_t nnz = R->row_start[nrays];
+        std::vector<int32_t> all_nodes(nnz);
+        std::vector<float> all_len(nnz);
+        for (int r = 0; r < nrays; r++) {
+            if (R->row_start[r + 1] == R->row_start[r]) continue;
+            const auto &nn = nodes_parts[slot[r]];
+            const auto &ll = len_parts[slot[r]];
+            std::copy(nn.begin(), nn.end(), all_nodes.begin() + R->row_start[r]);
+            std::copy(ll.begin(), ll.end(), all_len.begin() + R->row_start[r]);
+        }
+        R->nodes.alloc(std::max<int64_t>(1, nnz));
+        R->lengths.alloc(std::max<int64_t>(1, nnz));
+        if (nnz > 0) {
+            TS_HIP(hipMemcpy(R->nodes.p, all_nodes.data(), nnz * 4, hipMemcpyHostToDevice));
+            TS_HIP(hipMemcpy(R->lengths.p, all_len.data(), nnz * 4, hipMemcpyHostToDevice));
+        }
+        R->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = R.release();
+        return TS_OK;
+    });
+}
+
+int ts_rays_rows(ts_rays *r, int32_t *counts, int64_t *row_start, int32_t *nodes, float *lengths, double *tcalc_grad,
+                 int32_t *path_points)
+{
+    if (!r) return TS_EINVAL;
+    return guarded(nullptr, 0, [&]() -> int {
+        if (counts) std::copy(r->counts.begin(), r->counts.end(), counts);
+        if (row_start) std::copy(r->row_start.begin(), r->row_start.end(), row_start);
+        const int64_t nnz = r->row_start.empty() ? 0 : r->row_start.back();
+        if (nnz > 0 && nodes) TS_HIP(hipMemcpy(nodes, r->nodes.p, nnz * 4, hipMemcpyDeviceToHost));
+        if (nnz > 0 && lengths) TS_HIP(hipMemcpy(lengths, r->lengths.p, nnz * 4, hipMemcpyDeviceToHost));
+        if (tcalc_grad) std::copy(r->tcalc_grad.begin(), r->tcalc_grad.end(), tcalc_grad);
+        if (path_points) std::copy(r->path_points.begin(), r->path_points.end(), path_points);
+        return TS_OK;
+    });
+}
+
+int ts_rays_stats(ts_rays *r, int64_t *nnz, double *seconds)
+{
+    if (!r) return TS_EINVAL;
+    if (nnz) *nnz = r->row_start.empty() ? 0 : r->row_start.back();
+    if (seconds) *seconds = r->seconds;
+    return TS_OK;
+}
+
+void ts_rays_release(ts_rays *r)
+{
+    if (!r) return;
+    (void)hipSetDevice(r->ctx->device);
+    delete r;
+}
+
+}  // extern "C"
