@@ -26,7 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_UPDATE_FP32 = 12.0  # read own time, write it, read slowness; neighbours from cache
+# algorithmic bytes of one node update: read and write its own time, read its slowness (neighbours
+# are counted as served from cache): FP32 times 4+4+4, FP64 times 8+8+4
+BYTES_PER_UPDATE_FP32 = 12.0
+BYTES_PER_UPDATE_FP64 = 20.0
 
 
 def log(msg: str) -> None:
@@ -86,24 +89,27 @@ def main() -> int:
         rc, why = ts.solve("fast_fp32")
         if rc != 0:
             raise RuntimeError(f"solve declined: {why}")
-        log(f"warmup {w}: rounds {ts.rounds}, {ts.stats()[1]:.3f} s")
-    evals = ts.stats()[0]
+        log(f"warmup {w}: rounds {ts.rounds}, {ts.stats()['seconds']:.3f} s")
+    st0 = ts.stats()
+    e32, e64 = st0["updates_fp32"], st0["updates_fp64"]
+    evals = e32 + e64
+    alg_bytes = e32 * BYTES_PER_UPDATE_FP32 + e64 * BYTES_PER_UPDATE_FP64
     lat, sweep = [], []
     t_start = time.perf_counter()
     for k in range(args.steps):
         rc, why = ts.solve("fast_fp32")
         if rc != 0:
             raise RuntimeError(f"solve declined: {why}")
-        e, wall, sw = ts.stats()
-        lat.append(wall)
-        sweep.append(sw)
-        log(f"step {k}: {wall:.3f} s (sweep graphs {sw:.3f} s), rounds {ts.rounds}")
+        st = ts.stats()
+        lat.append(st["seconds"])
+        sweep.append(st["sweep_seconds"])
+        log(f"step {k}: {st['seconds']:.3f} s (sweep graphs {st['sweep_seconds']:.3f} s), rounds {ts.rounds}")
     total = time.perf_counter() - t_start
     ms_per_step = 1e3 * total / args.steps
     value = nf * N / (total / args.steps) / 1e9
     upd_rate = evals / (total / args.steps) / 1e9
     sweep_avg = float(np.mean(sweep))
-    achieved = evals * BYTES_PER_UPDATE_FP32 / sweep_avg / 1e9
+    achieved = alg_bytes / sweep_avg / 1e9
 
     # plain FP64 one-plane-per-launch form on a bounded sample (one of its batches)
     pf = min(args.plain_fields, nf)
@@ -133,7 +139,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+f64 (first-order phase on FP32 times, second-order phase on FP64 times)",
         "data": "synthetic (seeded Gaussian random field + ellipsoidal bodies)",
         "config": {"workload": f"batched eikonal: {nf} fields ({len(wl.stations)} stations x P,S) on "
                                f"{wl.grid.nx}x{wl.grid.ny}x{wl.grid.nz} at 0.5 km, one solve per step",
@@ -145,8 +151,9 @@ def main() -> int:
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "eik_fast_plane (sweep graph)",
-                     "bytes_per_unit": BYTES_PER_UPDATE_FP32, "units_per_step": evals,
+                     "kernel": "eik_fast_plane (sweep graphs)",
+                     "bytes_per_unit": {"fp32_update": BYTES_PER_UPDATE_FP32, "fp64_update": BYTES_PER_UPDATE_FP64},
+                     "units_per_step": {"fp32_updates": e32, "fp64_updates": e64},
                      "kernel_seconds_per_step": sweep_avg},
         "cpu_baseline": cpu,
     }

@@ -41,7 +41,8 @@ extern "C" {
 /* eikonal modes */
 #define TS_EIK_PLAIN_FP64 0 /* the reference scheme: FP64, one launch per diagonal plane, batch-level
                                convergence, host flag read per round (EikonalOpenCl.cs:168-343) */
-#define TS_EIK_FAST_FP32 1  /* batched: FP32 times in the offset form, 64 fields per wavefront, per-field
+#define TS_EIK_FAST_FP32 1  /* batched, mixed precision: the first-order phase on FP32 times (offset form),
+                               the second-order phase on FP64 times; 64 fields per wavefront, per-field
                                convergence, wavefront-ballot wake flags, one graph launch per round */
 #define TS_EIK_FAST_FP64 2  /* the batched schedule with FP64 times: equal to SphericalEikonal.Relax of
                                each field to the last bit */
@@ -116,9 +117,11 @@ int ts_tables_read(ts_tableset *t, int first, int count, double *out);
 int ts_tables_read_f32(ts_tableset *t, int first, int count, float *out);
 /* Per-field rounds: rounds[2*f] (first order), rounds[2*f+1] (second order) of the last solve. */
 int ts_tables_rounds(ts_tableset *t, int32_t *rounds);
-/* Node updates (stencil evaluations) the solve executed (counted on the first fast solve of the set),
- * the last solve's wall time, and the device time of its sweep graphs (HIP events on its stream). */
-int ts_tables_stats(ts_tableset *t, double *evaluations, double *seconds, double *sweep_seconds);
+/* Node updates (stencil evaluations) a fast solve executes on FP32 and on FP64 times (counted on the
+ * first fast solve of the set), the last solve's wall time, and the device time of its sweep graphs
+ * (HIP events on its stream). */
+int ts_tables_stats(ts_tableset *t, double *evaluations_fp32, double *evaluations_fp64, double *seconds,
+                    double *sweep_seconds);
 void ts_tables_release(ts_tableset *t);
 
 /* ---------------------------------------------------------------- rays

@@ -2,7 +2,7 @@
 the C ABI.
 
 Bars: the FP64 forms reproduce Relax to the last bit (the same update arithmetic in the same order,
-fields solved in the reference's plane schedule); the FP32 form stays within 1e-4 relative error of
+fields solved in the reference's plane schedule); the mixed form (FP32 first order, FP64 second order) stays within 1e-4 relative error of
 Relax at every computed node (measured errors are far below it); the plain
 form meets the reference's device gate, <= 1e-6 of the field range (EikonalOpenCl.cs:45).
 """
@@ -199,3 +199,23 @@ def test_duplicate_seeds_keep_the_smallest_time(device, oracle):
     assert ts.solve("fast_fp64")[0] == 0
     ref = oracle.relax(_og(oracle, g), s, dup_idx, dup_t)[0]
     assert np.array_equal(ts.read()[0], ref)
+
+
+def test_full_size_batched_mixed_against_fp64(device):
+    """The 400-field, 201 x 201 x 101 batch: the mixed form against the FP64 form (which equals Relax to
+    the last bit, as the tests above show on every field they check), on every tenth field."""
+    from tomostar_amd import eikonal as E
+    from tomostar_amd import workloads as W
+
+    wl = W.batched_eikonal_case(seed=1, stations=200)
+    seeds, mof = W.station_seed_sets(wl, E.point_source)
+    ts = E.TableSet(device, wl.grid, [wl.slowness_p, wl.slowness_s], seeds, mof)
+    picks = list(range(0, len(seeds), 10))
+    assert ts.solve("fast_fp64")[0] == 0
+    ref = np.stack([ts.read(f, 1)[0] for f in picks])
+    assert ts.solve("fast_fp32")[0] == 0
+    dev = np.stack([ts.read(f, 1)[0] for f in picks])
+    err = _rel_err(dev, ref, [seeds[f] for f in picks])
+    print(f"max relative error of the mixed form over {len(picks)} fields: {err:.3e}")
+    assert err < FP32_TOL
+    ts.release()

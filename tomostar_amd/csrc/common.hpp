@@ -25,6 +25,8 @@ struct ts_context {
     double eik_selftest = NAN;
     double lsqr_selftest = NAN;
     double lsqr_fast_selftest = NAN;
+    bool use_graphs = true;  // TOMOSTAR_NO_GRAPHS=1: launch kernels one by one (profilers that trace
+                             // graph launches poorly)
     std::mutex gate;  // one solve at a time per handle, as the reference's lock(_gate)
 };
 

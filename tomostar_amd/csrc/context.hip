@@ -143,6 +143,8 @@ int ts_hip_create(int device, int flags, ts_context **out, char *reason, int rea
         ctx->global_mem = prop.totalGlobalMem;
         ctx->cu_count = prop.multiProcessorCount;
         TS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        const char *ng = getenv("TOMOSTAR_NO_GRAPHS");
+        ctx->use_graphs = !(ng && ng[0] == '1');
         if (!(flags & TS_CREATE_SKIP_SELFTEST)) {
             char msg[256];
             ctx->eik_selftest = eikonal_selftest(ctx.get(), msg, sizeof msg);

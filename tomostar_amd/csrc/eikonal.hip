@@ -10,15 +10,17 @@
 //   plain (TS_EIK_PLAIN_FP64): the reference form on this GPU - FP64 times, one field per work item,
 //     one launch per diagonal plane, a per-(node,field) wake byte, batch-level convergence read back
 //     by the host after every round. It is the baseline the fast form is measured against.
-//   fast (TS_EIK_FAST_FP32 / TS_EIK_FAST_FP64): all fields of a table set in one pass. A wavefront
+//   fast (TS_EIK_FAST_FP32 mixed / TS_EIK_FAST_FP64): all fields of a table set in one pass. A wavefront
 //     updates one node for 64 fields, one field per lane, over times stored node-major in groups of
 //     64 fields ([group][node][64]), so every neighbour read is one contiguous 256-byte (FP32) line;
 //     the diagonal planes of all groups go out as one graph per round; each field keeps its own phase
 //     and round count (so every field follows Relax exactly), and a wave skips a node when no node it
 //     reads moved since its last update (one wake byte per node and group, raised with a ballot).
-//     FP32 times use the offset form of the update (every term of the quadratic is a cell crossing
-//     time measured from the smallest upwind neighbour), which keeps the error at the rounding of the
-//     stored time. FP64 times use Relax's arithmetic term for term and agree with it to the last bit.
+//     The mixed form runs the first-order phase on FP32 times in the offset form of the update (every
+//     term of the quadratic is a cell crossing time measured from the smallest upwind neighbour; the
+//     first-order fixed point is unique and FP32 reaches it to rounding) and the second-order phase
+//     on FP64 times; the FP64 form runs both phases in FP64 with Relax's arithmetic term for term and
+//     agrees with it to the last bit.
 #include <algorithm>
 #include <chrono>
 #include <memory>
@@ -294,10 +296,12 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int di
 
 // Per-field bookkeeping after a round (one wave per group): a field whose round moved no node by
 // more than the tolerance ends its phase (first order -> second order -> done); a field that
-// reaches the round cap in a phase is unsettled. state: 0/1 phase, 2 done, 3 unsettled, 4 unused.
+// reaches the round cap in a phase is unsettled. state: 0/1 phase, 2 done, 3 unsettled, 4 unused,
+// 5 first order settled and held (hold != 0: the second-order phase starts for all fields at once,
+// in FP64, once every field has settled its first-order phase in FP32).
 __global__ __launch_bounds__(64) void eik_fast_round_end(uint64_t *active, uint64_t *phase1, uint64_t *changed,
                                                          int32_t *state, int32_t *rounds, int cap,
-                                                         int32_t *status, uint8_t *reset)
+                                                         int32_t *status, uint8_t *reset, int hold)
 {
     const int g = blockIdx.x, lane = threadIdx.x, f = g * 64 + lane;
     int st = state[f];
@@ -306,8 +310,10 @@ __global__ __launch_bounds__(64) void eik_fast_round_end(uint64_t *active, uint6
     if (st < 2) {
         int r = ++rounds[2 * f + st];
         if (!ch) {
-            if (st == 0) { st = 1; need_reset = true; }
-            else st = 2;
+            if (st == 0) {
+                st = hold ? 5 : 1;
+                need_reset = !hold;
+            } else st = 2;
         } else if (r >= cap) st = 3;
         state[f] = st;
     }
@@ -330,6 +336,28 @@ __global__ void eik_fast_reset_dirty(uint8_t *dirty, const uint8_t *reset, int N
     if (!reset[g]) return;
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < (size_t)N) dirty[(size_t)g * N + q] = 1;
+}
+
+// Second-order phase of the mixed form: the FP32 first-order times become the FP64 starting point.
+__global__ void eik_mixed_widen(const float *T32, double *T64, size_t n)
+{
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) T64[q] = (double)T32[q];
+}
+
+__global__ __launch_bounds__(64) void eik_mixed_promote(uint64_t *active, uint64_t *phase1, uint64_t *changed,
+                                                        int32_t *state)
+{
+    const int g = blockIdx.x, lane = threadIdx.x, f = g * 64 + lane;
+    int st = state[f];
+    if (st == 5) st = 1;
+    state[f] = st;
+    const uint64_t act = __ballot(st == 1);
+    if (lane == 0) {
+        active[g] = act;
+        phase1[g] = act;
+        changed[g] = 0;
+    }
 }
 
 template <typename R>
@@ -407,7 +435,7 @@ struct ts_tableset {
     dbuf<double> tab64;                   // [fields][N] when the last solve ran in FP64
     bool have64 = false, solved = false;
     std::vector<int32_t> rounds;          // 2 per field
-    double evaluations = 0, seconds = 0, sweep_seconds = 0;
+    double evaluations = 0, evaluations32 = 0, evaluations64 = 0, seconds = 0, sweep_seconds = 0;
     // fast-form working storage, kept across solves
     dbuf<float> T32;
     dbuf<double> T64;
@@ -415,13 +443,13 @@ struct ts_tableset {
     dbuf<uint64_t> seedmask, active, phase1, changed;
     dbuf<int32_t> model_dev, state, rounds_dev, status;
     dbuf<unsigned long long> evals;
-    hipGraphExec_t graph = nullptr;
-    int graph_mode = -1;
-    bool graph_counting = false;
+    hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [FP64][counting]
     std::mutex gate;
     ~ts_tableset()
     {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto &row : graphs)
+            for (auto &gx : row)
+                if (gx) (void)hipGraphExecDestroy(gx);
     }
 };
 
@@ -575,7 +603,7 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
     A.active = t.active.p;
     A.phase1 = t.phase1.p;
     A.changed = t.changed.p;
-    A.evals = t.evals.p;
+    A.evals = t.evals.p + (sizeof(R) == 8 ? 1024 : 0);  // FP32 and FP64 updates counted apart
     A.planes = t.planes.p;
     A.nx = t.grid.nx; A.ny = t.grid.ny; A.nz = t.grid.nz; A.N = t.N;
     return A;
@@ -584,7 +612,7 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
 constexpr int kWPB = 4;
 
 template <typename R, bool kCount>
-void launch_round(ts_tableset &t, R *T, hipStream_t st)
+void launch_round(ts_tableset &t, R *T, hipStream_t st, int hold)
 {
     const int groups = (t.nfields + kLanes - 1) / kLanes;
     FastArgs<R> A = fast_args<R>(t, T);
@@ -596,27 +624,84 @@ void launch_round(ts_tableset &t, R *T, hipStream_t st)
             eik_fast_plane<R, kWPB, kCount><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt);
         }
     eik_fast_round_end<<<groups, 64, 0, st>>>(t.active.p, t.phase1.p, t.changed.p, t.state.p, t.rounds_dev.p,
-                                              sweep_round_cap(t.grid), t.status.p, t.reset.p);
+                                              sweep_round_cap(t.grid), t.status.p, t.reset.p, hold);
     dim3 rg(blocks_for(t.N, 256), groups);
     eik_fast_reset_dirty<<<rg, 256, 0, st>>>(t.dirty.p, t.reset.p, t.N);
 }
 
+// One round as a graph: 8 orderings x all planes x all groups, then the per-field bookkeeping. The
+// FP32 graph holds fields at the end of their first-order phase (mixed form), the FP64 one does not.
 template <typename R>
+hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count)
+{
+    hipGraphExec_t &ge = t.graphs[sizeof(R) == 8][count];
+    if (ge) return ge;
+    hipStream_t st = t.ctx->stream;
+    hipGraph_t gr;
+    TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int hold = sizeof(R) == 4;
+    if (count) launch_round<R, true>(t, T, st, hold);
+    else launch_round<R, false>(t, T, st, hold);
+    TS_HIP(hipStreamEndCapture(st, &gr));
+    hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    TS_HIP(e);
+    return ge;
+}
+
+// Rounds of one graph until no field is relaxing; the sweep time is bracketed by HIP events on the
+// solve's stream. Declines when a field exhausts its round cap.
+template <typename R>
+void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel, int32_t *h_status)
+{
+    hipGraphExec_t ge = t.ctx->use_graphs ? round_graph<R>(t, T, count) : nullptr;
+    const int hold = sizeof(R) == 4;
+    hipStream_t st = t.ctx->stream;
+    hipEvent_t ev0, ev1;
+    TS_HIP(hipEventCreate(&ev0));
+    TS_HIP(hipEventCreate(&ev1));
+    struct FreeEv { hipEvent_t a, b; ~FreeEv() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } free_ev{ev0, ev1};
+    const int max_rounds = 2 * sweep_round_cap(t.grid) + 2;
+    for (int round = 0; round < max_rounds; round++) {
+        if (cancelled(cancel)) fail(TS_CANCELLED, "cancelled");
+        TS_HIP(hipMemsetAsync(t.status.p, 0, 2 * sizeof(int32_t), st));
+        TS_HIP(hipEventRecord(ev0, st));
+        if (ge) TS_HIP(hipGraphLaunch(ge, st));
+        else if (count) launch_round<R, true>(t, T, st, hold);
+        else launch_round<R, false>(t, T, st, hold);
+        TS_HIP(hipEventRecord(ev1, st));
+        TS_HIP(hipMemcpyAsync(h_status, t.status.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        TS_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        TS_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        t.sweep_seconds += ms * 1e-3;
+        if (h_status[1] > 0) fail(TS_DECLINED, "sweeping not settled after %d rounds", sweep_round_cap(t.grid));
+        if (h_status[0] == 0) return;
+    }
+    fail(TS_DECLINED, "sweeping not settled");
+}
+
+// The fast forms. FP64 (mode TS_EIK_FAST_FP64): Relax per field, both phases in FP64. Mixed (mode
+// TS_EIK_FAST_FP32): the first-order phase in FP32 offset form (its fixed point is unique, and FP32
+// reaches it to rounding), then the second-order phase in FP64 from the widened FP32 times - the
+// second-order promotion amplifies storage rounding (an FP32 second-order phase drifts by up to
+// 4e-4 of the time on a 0.5 km grid), so it keeps FP64 times.
 void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool count)
 {
     ts_context *ctx = t.ctx;
     hipStream_t st = ctx->stream;
+    const bool mixed = mode == TS_EIK_FAST_FP32;
     const int N = t.N, groups = (t.nfields + kLanes - 1) / kLanes, slots = groups * kLanes;
     const size_t nodes_total = (size_t)groups * N;
-    dbuf<R> &Tb = [&]() -> dbuf<R> & { if constexpr (sizeof(R) == 4) return t.T32; else return t.T64; }();
-    Tb.ensure(nodes_total * kLanes);
+    if (mixed) t.T32.ensure(nodes_total * kLanes);
+    t.T64.ensure(nodes_total * kLanes);
     t.dirty.ensure(nodes_total);
     t.seedmask.ensure(nodes_total);
     t.active.ensure(groups); t.phase1.ensure(groups); t.changed.ensure(groups); t.reset.ensure(groups);
     t.model_dev.ensure(slots); t.state.ensure(slots); t.rounds_dev.ensure(2 * slots); t.status.ensure(2);
-    t.evals.ensure(1024);
+    t.evals.ensure(2048);
     t.tab32.ensure((size_t)t.nfields * N);
-    if (sizeof(R) == 8) t.tab64.ensure((size_t)t.nfields * N);
+    t.tab64.ensure((size_t)t.nfields * N);
 
     // per-field state: all fields start in the first-order phase
     std::vector<int32_t> model(slots, 0), state(slots, 4);
@@ -632,82 +717,61 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     TS_HIP(hipMemsetAsync(t.phase1.p, 0, groups * 8, st));
     TS_HIP(hipMemsetAsync(t.changed.p, 0, groups * 8, st));
     TS_HIP(hipMemsetAsync(t.rounds_dev.p, 0, 2 * slots * 4, st));
-    TS_HIP(hipMemsetAsync(t.evals.p, 0, 1024 * 8, st));
-    eik_fast_clear<R><<<blocks_for(nodes_total * kLanes, 256), 256, 0, st>>>(Tb.p, t.dirty.p, t.seedmask.p, nodes_total);
-    {
-        std::vector<int32_t> sf, sn;
-        std::vector<double> sv;
-        for (int f = 0; f < t.nfields; f++)
-            for (size_t q = 0; q < t.seed_node[f].size(); q++) {
-                sf.push_back(f);
-                sn.push_back(t.seed_node[f][q]);
-                sv.push_back(t.seed_time[f][q]);
-            }
-        dbuf<int32_t> df(sf.size()), dn(sn.size());
-        dbuf<double> dv(sv.size());
-        TS_HIP(hipMemcpyAsync(df.p, sf.data(), sf.size() * 4, hipMemcpyHostToDevice, st));
-        TS_HIP(hipMemcpyAsync(dn.p, sn.data(), sn.size() * 4, hipMemcpyHostToDevice, st));
-        TS_HIP(hipMemcpyAsync(dv.p, sv.data(), sv.size() * 8, hipMemcpyHostToDevice, st));
-        eik_fast_seed<R><<<blocks_for(sf.size(), 256), 256, 0, st>>>(Tb.p, t.seedmask.p, df.p, dn.p, dv.p,
-                                                                      (int)sf.size(), N);
-        TS_HIP(hipGetLastError());
-        TS_HIP(hipStreamSynchronize(st));
+    TS_HIP(hipMemsetAsync(t.evals.p, 0, 2048 * 8, st));
+    std::vector<int32_t> sf, sn;
+    std::vector<double> sv;
+    for (int f = 0; f < t.nfields; f++)
+        for (size_t q = 0; q < t.seed_node[f].size(); q++) {
+            sf.push_back(f);
+            sn.push_back(t.seed_node[f][q]);
+            sv.push_back(t.seed_time[f][q]);
+        }
+    dbuf<int32_t> df(sf.size()), dn(sn.size());
+    dbuf<double> dv(sv.size());
+    TS_HIP(hipMemcpyAsync(df.p, sf.data(), sf.size() * 4, hipMemcpyHostToDevice, st));
+    TS_HIP(hipMemcpyAsync(dn.p, sn.data(), sn.size() * 4, hipMemcpyHostToDevice, st));
+    TS_HIP(hipMemcpyAsync(dv.p, sv.data(), sv.size() * 8, hipMemcpyHostToDevice, st));
+    const unsigned cb = blocks_for(nodes_total * kLanes, 256), sb = blocks_for(sf.size(), 256);
+    if (mixed) {
+        eik_fast_clear<float><<<cb, 256, 0, st>>>(t.T32.p, t.dirty.p, t.seedmask.p, nodes_total);
+        eik_fast_seed<float><<<sb, 256, 0, st>>>(t.T32.p, t.seedmask.p, df.p, dn.p, dv.p, (int)sf.size(), N);
+    } else {
+        eik_fast_clear<double><<<cb, 256, 0, st>>>(t.T64.p, t.dirty.p, t.seedmask.p, nodes_total);
+        eik_fast_seed<double><<<sb, 256, 0, st>>>(t.T64.p, t.seedmask.p, df.p, dn.p, dv.p, (int)sf.size(), N);
     }
-    // One round as a graph (8 orderings x all planes x all groups, then the per-field bookkeeping).
-    if (!t.graph || t.graph_mode != mode || t.graph_counting != count) {
-        if (t.graph) { TS_HIP(hipGraphExecDestroy(t.graph)); t.graph = nullptr; }
-        hipGraph_t gr;
-        TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        if (count) launch_round<R, true>(t, Tb.p, st);
-        else launch_round<R, false>(t, Tb.p, st);
-        TS_HIP(hipStreamEndCapture(st, &gr));
-        hipError_t e = hipGraphInstantiate(&t.graph, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        TS_HIP(e);
-        t.graph_mode = mode;
-        t.graph_counting = count;
-    }
+    TS_HIP(hipGetLastError());
+    TS_HIP(hipStreamSynchronize(st));
     int32_t *h_status = nullptr;
     TS_HIP(hipHostMalloc((void **)&h_status, 2 * sizeof(int32_t), hipHostMallocDefault));
     struct Free { int32_t *p; ~Free() { if (p) (void)hipHostFree(p); } } free_h{h_status};
-    const int max_rounds = 2 * sweep_round_cap(t.grid) + 2;
-    bool done = false;
-    // HIP events on the solve's stream bracket each round's graph: the sweep time of the solve
-    hipEvent_t ev0, ev1;
-    TS_HIP(hipEventCreate(&ev0));
-    TS_HIP(hipEventCreate(&ev1));
-    struct FreeEv { hipEvent_t a, b; ~FreeEv() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } free_ev{ev0, ev1};
     t.sweep_seconds = 0;
-    for (int round = 0; round < max_rounds; round++) {
-        if (cancelled(cancel)) fail(TS_CANCELLED, "cancelled");
-        TS_HIP(hipMemsetAsync(t.status.p, 0, 2 * sizeof(int32_t), st));
-        TS_HIP(hipEventRecord(ev0, st));
-        TS_HIP(hipGraphLaunch(t.graph, st));
-        TS_HIP(hipEventRecord(ev1, st));
-        TS_HIP(hipMemcpyAsync(h_status, t.status.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        TS_HIP(hipStreamSynchronize(st));
-        float ms = 0;
-        TS_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-        t.sweep_seconds += ms * 1e-3;
-        if (h_status[1] > 0) fail(TS_DECLINED, "sweeping not settled after %d rounds", sweep_round_cap(t.grid));
-        if (h_status[0] == 0) { done = true; break; }
+    if (mixed) {
+        run_rounds<float>(t, t.T32.p, count, cancel, h_status);
+        eik_mixed_widen<<<blocks_for(nodes_total * kLanes, 256), 256, 0, st>>>(t.T32.p, t.T64.p, nodes_total * kLanes);
+        // the fixed seed times again, unrounded
+        eik_fast_seed<double><<<sb, 256, 0, st>>>(t.T64.p, t.seedmask.p, df.p, dn.p, dv.p, (int)sf.size(), N);
+        eik_mixed_promote<<<groups, 64, 0, st>>>(t.active.p, t.phase1.p, t.changed.p, t.state.p);
+        eik_fill_u8<<<blocks_for(nodes_total, 256), 256, 0, st>>>(t.dirty.p, nodes_total, 1);
+        TS_HIP(hipGetLastError());
     }
-    if (!done) fail(TS_DECLINED, "sweeping not settled");
+    run_rounds<double>(t, t.T64.p, count, cancel, h_status);
     dim3 ug(blocks_for(N, 64), groups);
-    eik_fast_unpack<R><<<ug, 256, 0, st>>>(Tb.p, N, t.nfields, t.tab32.p, sizeof(R) == 8 ? t.tab64.p : nullptr);
+    eik_fast_unpack<double><<<ug, 256, 0, st>>>(t.T64.p, N, t.nfields, t.tab32.p, t.tab64.p);
     TS_HIP(hipGetLastError());
     std::vector<int32_t> rounds(2 * slots);
     TS_HIP(hipMemcpyAsync(rounds.data(), t.rounds_dev.p, rounds.size() * 4, hipMemcpyDeviceToHost, st));
-    std::vector<unsigned long long> ev(1024);
-    TS_HIP(hipMemcpyAsync(ev.data(), t.evals.p, 1024 * 8, hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> ev(2048);
+    TS_HIP(hipMemcpyAsync(ev.data(), t.evals.p, 2048 * 8, hipMemcpyDeviceToHost, st));
     TS_HIP(hipStreamSynchronize(st));
     for (int f = 0; f < t.nfields; f++) { t.rounds[2 * f] = rounds[2 * f]; t.rounds[2 * f + 1] = rounds[2 * f + 1]; }
     if (count) {
-        double e = 0;
-        for (auto v : ev) e += (double)v;
-        t.evaluations = e;
+        double e32 = 0, e64 = 0;
+        for (int q = 0; q < 1024; q++) { e32 += (double)ev[q]; e64 += (double)ev[1024 + q]; }
+        t.evaluations = e32 + e64;
+        t.evaluations32 = e32;
+        t.evaluations64 = e64;
     }
-    t.have64 = sizeof(R) == 8;
+    t.have64 = true;
 }
 
 }  // namespace
@@ -837,8 +901,7 @@ int ts_tables_solve(ts_tableset *t, int mode, const volatile int32_t *cancel, in
         auto t0 = std::chrono::steady_clock::now();
         bool count = t->evaluations == 0;
         if (mode == TS_EIK_PLAIN_FP64) solve_plain(*t, cancel);
-        else if (mode == TS_EIK_FAST_FP32) solve_fast<float>(*t, mode, cancel, count);
-        else solve_fast<double>(*t, mode, cancel, count);
+        else solve_fast(*t, mode, cancel, count);
         t->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         t->solved = true;
         if (rounds_out) {
@@ -895,10 +958,12 @@ int ts_tables_rounds(ts_tableset *t, int32_t *rounds)
     return TS_OK;
 }
 
-int ts_tables_stats(ts_tableset *t, double *evaluations, double *seconds, double *sweep_seconds)
+int ts_tables_stats(ts_tableset *t, double *evaluations_fp32, double *evaluations_fp64, double *seconds,
+                    double *sweep_seconds)
 {
     if (!t) return TS_EINVAL;
-    if (evaluations) *evaluations = t->evaluations;
+    if (evaluations_fp32) *evaluations_fp32 = t->evaluations32;
+    if (evaluations_fp64) *evaluations_fp64 = t->evaluations64;
     if (seconds) *seconds = t->seconds;
     if (sweep_seconds) *sweep_seconds = t->sweep_seconds;
     return TS_OK;

@@ -534,14 +534,16 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     TS_HIP(hipGetLastError());
     TS_HIP(hipStreamSynchronize(st));
     // one iteration as a graph, replayed; the stop flags are read every `check` iterations
-    hipGraph_t gr;
     hipGraphExec_t ge = nullptr;
-    TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    fast_iteration<KP>(A, F, st, o.hist != nullptr);
-    TS_HIP(hipStreamEndCapture(st, &gr));
-    hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    TS_HIP(e);
+    if (A.ctx->use_graphs) {
+        hipGraph_t gr;
+        TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        fast_iteration<KP>(A, F, st, o.hist != nullptr);
+        TS_HIP(hipStreamEndCapture(st, &gr));
+        hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        TS_HIP(e);
+    }
     struct FreeG { hipGraphExec_t g; ~FreeG() { if (g) (void)hipGraphExecDestroy(g); } } fg{ge};
     hipEvent_t ev0, ev1;
     TS_HIP(hipEventCreate(&ev0));
@@ -554,7 +556,10 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     while (it < max_it) {
         if (cancelled(cancel)) fail(TS_CANCELLED, "cancelled");
         int todo = std::min(check, max_it - it);
-        for (int q = 0; q < todo; q++) TS_HIP(hipGraphLaunch(ge, st));
+        for (int q = 0; q < todo; q++) {
+            if (ge) TS_HIP(hipGraphLaunch(ge, st));
+            else fast_iteration<KP>(A, F, st, o.hist != nullptr);
+        }
         it += todo;
         TS_HIP(hipMemcpyAsync(cur.data(), F.S.p, KP * sizeof(LsqrScal), hipMemcpyDeviceToHost, st));
         TS_HIP(hipStreamSynchronize(st));

@@ -158,10 +158,10 @@ class TableSet:
         return out.reshape(self.nfields, 2)
 
     def stats(self):
-        """(node updates executed, last solve wall seconds, last solve sweep-graph device seconds)."""
-        e, s, k = C.c_double(), C.c_double(), C.c_double()
-        check(_lib.load().ts_tables_stats(self._h, C.byref(e), C.byref(s), C.byref(k)))
-        return e.value, s.value, k.value
+        """dict: node updates on FP32 and FP64 times, last solve wall seconds and sweep-graph device seconds."""
+        a, b, s, k = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        check(_lib.load().ts_tables_stats(self._h, C.byref(a), C.byref(b), C.byref(s), C.byref(k)))
+        return {"updates_fp32": a.value, "updates_fp64": b.value, "seconds": s.value, "sweep_seconds": k.value}
 
     def release(self):
         if self._h is not None:
