@@ -89,8 +89,8 @@ def test_plain_stop_rules_follow_the_oracle(device, oracle):
         # valid) paths whose residual estimates part by up to ~1e-6 before both converge
         assert np.allclose(r.residual_history, hist, rtol=1e-5, atol=0)
         assert abs(r.residual_history[-1] - hist[-1]) <= 1e-9 * hist[-1]
-        # the two paths meet at the same least-squares solution to ~1e-8 (2e-8 measured here)
-        assert np.linalg.norm(r.solution - x) <= 1e-7 * np.linalg.norm(x)
+        # the two paths meet at the same solution to the accuracy the stop rule asked for
+        assert np.linalg.norm(r.solution - x) <= max(1e-7, 10 * atol) * np.linalg.norm(x)
 
 
 @pytest.mark.parametrize("k", [1, 5, 32, 33])

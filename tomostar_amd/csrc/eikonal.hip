@@ -383,7 +383,8 @@ __global__ void eik_fast_seed(R *T, uint64_t *seedmask, const int32_t *field, co
 // [group][node][64] -> field-major tables (FP32, and FP64 when requested), through an LDS tile so
 // both the read and the write are coalesced.
 template <typename R>
-__global__ __launch_bounds__(256) void eik_fast_unpack(const R *T, int N, int nfields, float *out32, double *out64)
+__global__ __launch_bounds__(256) void eik_fast_unpack(const R *T, int N, const int32_t *field_of_slot, float *out32,
+                                                       double *out64)
 {
     __shared__ R tile[64][65];
     const int g = blockIdx.y, n0 = blockIdx.x * 64, t = threadIdx.x;
@@ -396,8 +397,8 @@ __global__ __launch_bounds__(256) void eik_fast_unpack(const R *T, int N, int nf
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        int fl = (t >> 6) + r * 4, f = g * kLanes + fl, node = n0 + (t & 63);
-        if (f >= nfields || node >= N) continue;
+        int fl = (t >> 6) + r * 4, f = field_of_slot[g * kLanes + fl], node = n0 + (t & 63);
+        if (f < 0 || node >= N) continue;
         R v = tile[t & 63][fl];
         size_t at = (size_t)f * N + node;
         if (out32) out32[at] = isfinite(v) ? (float)v : INFINITY;
@@ -422,6 +423,10 @@ struct ts_tableset {
     Grid grid{};
     int nmodels = 0, nfields = 0, N = 0;
     std::vector<int32_t> model_of_field;
+    // fields go to 64-lane groups in the Morton order of their sources, so the fields a wavefront
+    // relaxes together have nearby fronts and wake the same nodes
+    std::vector<int32_t> slot_of_field, field_of_slot;
+    dbuf<int32_t> field_of_slot_dev;
     std::vector<std::vector<int32_t>> seed_node;  // merged seeds of each field (smallest time wins)
     std::vector<std::vector<double>> seed_time;
     std::vector<double> hlon, hlat;
@@ -707,9 +712,10 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     std::vector<int32_t> model(slots, 0), state(slots, 4);
     std::vector<uint64_t> active(groups, 0);
     for (int f = 0; f < t.nfields; f++) {
-        model[f] = t.model_of_field[f];
-        state[f] = 0;
-        active[f / kLanes] |= 1ull << (f % kLanes);
+        const int sl = t.slot_of_field[f];
+        model[sl] = t.model_of_field[f];
+        state[sl] = 0;
+        active[sl / kLanes] |= 1ull << (sl % kLanes);
     }
     TS_HIP(hipMemcpyAsync(t.model_dev.p, model.data(), slots * 4, hipMemcpyHostToDevice, st));
     TS_HIP(hipMemcpyAsync(t.state.p, state.data(), slots * 4, hipMemcpyHostToDevice, st));
@@ -722,7 +728,7 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     std::vector<double> sv;
     for (int f = 0; f < t.nfields; f++)
         for (size_t q = 0; q < t.seed_node[f].size(); q++) {
-            sf.push_back(f);
+            sf.push_back(t.slot_of_field[f]);
             sn.push_back(t.seed_node[f][q]);
             sv.push_back(t.seed_time[f][q]);
         }
@@ -756,14 +762,18 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     }
     run_rounds<double>(t, t.T64.p, count, cancel, h_status);
     dim3 ug(blocks_for(N, 64), groups);
-    eik_fast_unpack<double><<<ug, 256, 0, st>>>(t.T64.p, N, t.nfields, t.tab32.p, t.tab64.p);
+    eik_fast_unpack<double><<<ug, 256, 0, st>>>(t.T64.p, N, t.field_of_slot_dev.p, t.tab32.p, t.tab64.p);
     TS_HIP(hipGetLastError());
     std::vector<int32_t> rounds(2 * slots);
     TS_HIP(hipMemcpyAsync(rounds.data(), t.rounds_dev.p, rounds.size() * 4, hipMemcpyDeviceToHost, st));
     std::vector<unsigned long long> ev(2048);
     TS_HIP(hipMemcpyAsync(ev.data(), t.evals.p, 2048 * 8, hipMemcpyDeviceToHost, st));
     TS_HIP(hipStreamSynchronize(st));
-    for (int f = 0; f < t.nfields; f++) { t.rounds[2 * f] = rounds[2 * f]; t.rounds[2 * f + 1] = rounds[2 * f + 1]; }
+    for (int f = 0; f < t.nfields; f++) {
+        const int sl = t.slot_of_field[f];
+        t.rounds[2 * f] = rounds[2 * sl];
+        t.rounds[2 * f + 1] = rounds[2 * sl + 1];
+    }
     if (count) {
         double e32 = 0, e64 = 0;
         for (int q = 0; q < 1024; q++) { e32 += (double)ev[q]; e64 += (double)ev[1024 + q]; }
@@ -822,6 +832,7 @@ int ts_tables_create(ts_context *ctx, const ts_grid *grid, int nmodels, const do
         return TS_EINVAL;
     }
     return guarded(reason, reason_len, [&]() -> int {
+        TS_HIP(hipSetDevice(ctx->device));
         std::unique_ptr<ts_tableset> t(new ts_tableset);
         t->ctx = ctx;
         t->grid = make_grid(*grid);
@@ -857,6 +868,39 @@ int ts_tables_create(ts_context *ctx, const ts_grid *grid, int nmodels, const do
                 t->seed_node[f].push_back(seeds[q].first);
                 t->seed_time[f].push_back(seeds[q].second);
             }
+        }
+        {
+            // Morton code of each field's first seed node (its source), ties kept in field order
+            auto spread = [](uint32_t v) {
+                uint64_t x = v & 0x3ff;
+                x = (x | (x << 16)) & 0x30000ff;
+                x = (x | (x << 8)) & 0x300f00f;
+                x = (x | (x << 4)) & 0x30c30c3;
+                x = (x | (x << 2)) & 0x9249249;
+                return x;
+            };
+            const Grid &gg = t->grid;
+            std::vector<uint64_t> code(nfields);
+            for (int f = 0; f < nfields; f++) {
+                const int32_t n0 = t->seed_node[f][std::min_element(t->seed_time[f].begin(), t->seed_time[f].end()) -
+                                                    t->seed_time[f].begin()];
+                const int k = n0 / (gg.nx * gg.ny), rest = n0 - k * gg.nx * gg.ny, j = rest / gg.nx, i = rest - j * gg.nx;
+                const uint32_t si = (uint32_t)((int64_t)i * 1024 / gg.nx), sj = (uint32_t)((int64_t)j * 1024 / gg.ny),
+                               sk = (uint32_t)((int64_t)k * 1024 / gg.nz);
+                code[f] = spread(si) | (spread(sj) << 1) | (spread(sk) << 2);
+            }
+            std::vector<int32_t> order(nfields);
+            for (int f = 0; f < nfields; f++) order[f] = f;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return code[a] < code[b]; });
+            const int slots = (nfields + kLanes - 1) / kLanes * kLanes;
+            t->slot_of_field.assign(nfields, 0);
+            t->field_of_slot.assign(slots, -1);
+            for (int q = 0; q < nfields; q++) {
+                t->slot_of_field[order[q]] = q;
+                t->field_of_slot[q] = order[q];
+            }
+            t->field_of_slot_dev.alloc(slots);
+            TS_HIP(hipMemcpy(t->field_of_slot_dev.p, t->field_of_slot.data(), slots * 4, hipMemcpyHostToDevice));
         }
         t->hlon.resize((size_t)t->grid.ny * t->grid.nz);
         t->hlat.resize(t->grid.nz);
