@@ -122,6 +122,10 @@ int ts_tables_rounds(ts_tableset *t, int32_t *rounds);
  * (HIP events on its stream). */
 int ts_tables_stats(ts_tableset *t, double *evaluations_fp32, double *evaluations_fp64, double *seconds,
                     double *sweep_seconds);
+/* The last fast solve's sweep rounds and sweep-graph device seconds split by the width of the times
+ * swept ([0] FP32, [1] FP64), and the plane kernels one round launches (8 orderings x planes). No
+ * counterpart in the reference: it is the figure a roofline of the sweep kernel is computed from. */
+int ts_tables_sweep_stats(ts_tableset *t, int32_t rounds[2], int32_t *plane_launches_per_round, double seconds[2]);
 void ts_tables_release(ts_tableset *t);
 
 /* ---------------------------------------------------------------- rays
@@ -152,7 +156,55 @@ int ts_matrix_create(ts_context *ctx, int rows, int cols, const int32_t *row_sta
                      const double *values, int keep_fp64, ts_matrix **out, char *reason, int reason_len);
 /* Non-zeros, and the device time of the last fast solve's iterations (HIP events on its stream). */
 int ts_matrix_stats(ts_matrix *a, int64_t *nnz, double *seconds);
+/* Rows, columns, non-zeros and the wall time of the upload or assembly (transpose included). */
+int ts_matrix_shape(ts_matrix *a, int32_t *rows, int32_t *cols, int64_t *nnz, double *build_seconds);
+/* The matrix back in host memory (CsrMatrix's arrays): row_start[rows+1], column[nnz], values[nnz]
+ * (FP64 when kept, else the FP32 values widened) and, for a matrix from ts_system_build, the
+ * right-hand side rhs[rows]. Any pointer may be NULL. */
+int ts_matrix_read(ts_matrix *a, int32_t *row_start, int32_t *column, double *values, double *rhs);
+/* The stored transpose (CsrMatrix.Transpose, SparseMatrix.cs:79-97): row_start[cols+1], column[nnz]
+ * (each transposed row's rows ascending) and its FP32 values. */
+int ts_matrix_read_transpose(ts_matrix *a, int32_t *row_start, int32_t *column, float *values);
 void ts_matrix_release(ts_matrix *a);
+
+/* ---------------------------------------------------------------- the linear system
+ * TravelTimeTomography.BuildSystem (Tomography/TravelTimeTomography.cs:860-1034) on a plain node grid
+ * (no parameter lattice, adaptive mesh or layered model), assembled on the device from ray rows
+ * resident in HBM into an LSQR matrix. Settings follow TomographySettings (TravelTimeTomography.cs:
+ * 30-110); any_p / any_s say whether the data hold P / S arrivals at all (rejected ones included,
+ * :867-868 and :879-882), which fixes the column layout. */
+typedef struct {
+    int32_t vpvs;                /* VelocityParameterization.VpVpVs: S rows on (P, Vp/Vs) columns */
+    int32_t invert_p, invert_s;  /* InvertP, InvertS */
+    int32_t any_p, any_s;
+    int32_t joint_hypocentres;   /* JointHypocentres: 4 columns per event */
+    int32_t station_corrections; /* StationCorrections: one column per station and phase */
+    int32_t gradient_smoothing;  /* 0: SmoothingMethod.Laplacian, 1: first differences (TotalVariation
+                                    without reweighting) */
+    int32_t smoothing_in_nodes;  /* SmoothingScale.Nodes (axis weights 1, 1, vertical), else Kilometres */
+    int32_t nevents, nstations;
+    double damping, smoothing;   /* the iteration's damping and smoothing weights */
+    double vertical_weight;      /* VerticalSmoothingWeight */
+    double damping_hypocentre;   /* DampingHypocentre */
+    double damping_station;      /* DampingStation */
+    double s_factor;             /* SRegularisationFactor (<= 0: 1) */
+    double absolute_weight;      /* weight of the absolute-time rows (> 0 adds them) */
+} ts_system_opts;
+
+/* rays: the traced rows of the arrivals (ts_rays_trace). Data row r of the system is arrival
+ * ray_of_row[r] (in the order given: the reference adds the used rows in arrival order, :953-972),
+ * with its phase (0 P, 1 S), event, station, residual (observed - calculated, s) and sigma (s).
+ * event_fixed / event_outside [nevents]: Event.Fixed / Event.Outside. sP, sS: the current slowness
+ * models; sRefP, sRefS: the reference models; kRef: the reference Vp/Vs (sRefS / sRefP); all [nodes
+ * of grid]. Output: the matrix (FP32 values, and FP64 when keep_fp64), its right-hand side
+ * (ts_matrix_read) and layout[8] = {offP, offS, offH, offCP, offCS (-1: absent), columns, rows, data
+ * rows} (MatrixLayout, :1031). */
+int ts_system_build(ts_context *ctx, const ts_grid *grid, const ts_system_opts *opts, ts_rays *rays, int nrows,
+                    const int32_t *ray_of_row, const int32_t *phase, const int32_t *event, const int32_t *station,
+                    const double *residual, const double *sigma, const uint8_t *event_fixed,
+                    const uint8_t *event_outside, const double *sP, const double *sS, const double *sRefP,
+                    const double *sRefS, const double *kRef, int keep_fp64, ts_matrix **out, int64_t layout[8],
+                    char *reason, int reason_len);
 
 /* Lsqr.Solve (Numerics/Lsqr.cs:34-152) / LsqrOpenCl.TrySolve (LsqrOpenCl.cs:244-270) for k (1..64)
  * right-hand sides b[k*m] (RHS-major), each with its own damping damp[k]: minimises

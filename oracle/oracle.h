@@ -84,6 +84,21 @@ int or_lsqr(int m, int n, const int64_t *rs, const int *ci, const double *va,
             int max_it, double atol, double btol, double conlim, int threads,
             double *x, double stats[4], double *rnorm_hist);
 
+/* --- the linear system (Tomography/TravelTimeTomography.cs:860-1034 BuildSystem, plain node grid) --- */
+typedef struct {
+    int vpvs, joint_hypocentres, station_corrections, invert_p, invert_s, gradient_smoothing;
+    int nevents, nstations, any_p, any_s, smoothing_in_nodes;
+    double damping, smoothing, vertical_weight, damping_hypocentre, damping_station, s_factor, absolute_weight;
+} or_system_opts;
+
+int or_build_system(const or_griddef *g, const or_system_opts *o, int nrows, const int64_t *row_start,
+                    const int *row_nodes, const float *row_len, const double *row_tg, const int *phase,
+                    const int *event, const int *station, const double *residual, const double *sigma,
+                    const unsigned char *ev_fixed, const unsigned char *ev_outside, const double *sP,
+                    const double *sS, const double *sRefP, const double *sRefS, const double *kRef,
+                    int64_t cap_rows, int64_t cap_nnz, int64_t *rs, int *ci, double *va, double *rhs,
+                    int64_t dims[4]);
+
 #ifdef __cplusplus
 }
 #endif

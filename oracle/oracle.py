@@ -56,6 +56,7 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _G = C.POINTER(GridDef)
 
 
@@ -334,3 +335,51 @@ def ray_rows_batch(fwd: GridDef, tables, stations, inv: GridDef, ray_table, even
             a = r * cap_per_ray
             out.append((nodes[a:a + c].copy(), lens[a:a + c].copy(), *tg[4 * r:4 * r + 4]))
     return out
+
+
+class SystemOpts(C.Structure):
+    """TomographySettings fields BuildSystem reads (Tomography/TravelTimeTomography.cs:30-110)."""
+
+    _fields_ = [("vpvs", C.c_int), ("joint_hypocentres", C.c_int), ("station_corrections", C.c_int),
+                ("invert_p", C.c_int), ("invert_s", C.c_int), ("gradient_smoothing", C.c_int),
+                ("nevents", C.c_int), ("nstations", C.c_int), ("any_p", C.c_int), ("any_s", C.c_int),
+                ("smoothing_in_nodes", C.c_int),
+                ("damping", C.c_double), ("smoothing", C.c_double), ("vertical_weight", C.c_double),
+                ("damping_hypocentre", C.c_double), ("damping_station", C.c_double), ("s_factor", C.c_double),
+                ("absolute_weight", C.c_double)]
+
+
+def build_system(g: GridDef, opts: dict, row_start, nodes, lens, tcalc_grad, phase, event, station, residual, sigma,
+                 ev_fixed, ev_outside, sP, sS, sRefP, sRefS, kRef):
+    """BuildSystem restated (plain node grid). Rows are the used data rows in order: row_start/nodes/lens
+    their ray rows, tcalc_grad[r] = (TCalc, Ge, Gn, Gz). Returns (row_start, column, values, rhs, dims)
+    with dims = (rows, cols, nnz, data rows)."""
+    L = lib()
+    if not getattr(L, "_sys_declared", False):
+        L.or_build_system.restype = C.c_int
+        L.or_build_system.argtypes = [_G, C.POINTER(SystemOpts), C.c_int, _lp, _ip, _fp, _dp, _ip, _ip, _ip, _dp, _dp,
+                                      _u8, _u8, _dp, _dp, _dp, _dp, _dp, C.c_int64, C.c_int64, _lp, _ip, _dp, _dp, _lp]
+        L._sys_declared = True
+    o = SystemOpts(**{k: opts.get(k, 0) for k, _ in SystemOpts._fields_})
+    n = g.count
+    nrows = len(phase)
+    row_start = np.ascontiguousarray(row_start, np.int64)
+    nodes = np.ascontiguousarray(nodes, np.int32)
+    cap_rows = nrows + 8 * n + 4 * o.nevents + 2 * o.nstations + 4
+    cap_nnz = 2 * nodes.size + 6 * nrows + 30 * n + 4 * o.nevents + 4 * o.nstations + 8
+    rs = np.zeros(cap_rows + 1, np.int64)
+    ci = np.zeros(cap_nnz, np.int32)
+    va = np.zeros(cap_nnz)
+    rhs = np.zeros(cap_rows)
+    dims = np.zeros(4, np.int64)
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)
+    u8 = lambda a: np.ascontiguousarray(a, np.uint8) if len(a) else np.zeros(1, np.uint8)
+    rc = L.or_build_system(C.byref(g), C.byref(o), nrows, row_start, nodes, np.ascontiguousarray(lens, np.float32),
+                           f64(tcalc_grad).ravel() if nrows else np.zeros(4), i32(phase), i32(event), i32(station),
+                           f64(residual), f64(sigma), u8(ev_fixed), u8(ev_outside), f64(sP), f64(sS), f64(sRefP),
+                           f64(sRefS), f64(kRef), cap_rows, cap_nnz, rs, ci, va, rhs, dims)
+    if rc != 0:
+        raise RuntimeError("oracle system exceeds its capacity")
+    m, cols, nnz, data_rows = (int(x) for x in dims)
+    return rs[:m + 1].copy(), ci[:nnz].copy(), va[:nnz].copy(), rhs[:m].copy(), (m, cols, nnz, data_rows)

@@ -82,3 +82,26 @@ def test_invalid_arguments_are_rejected():
     assert L.ts_eik_suggest_batch(None, 1000) == _lib.TS_EINVAL
     r = _lib.reason_buffer()
     assert L.ts_tables_create(None, None, 1, None, 1, None, None, None, None, None, r, len(r)) == _lib.TS_EINVAL
+
+
+def test_system_build_marshalling_without_a_device():
+    """The Python front end of ts_system_build passes every argument of the C signature: with no
+    context the library rejects the call before touching a GPU."""
+    from types import SimpleNamespace
+
+    from tomostar_amd import _lib
+    from tomostar_amd import system as Sy
+    from tomostar_amd.grid import GridDefinition
+
+    g = GridDefinition(13.0, 13.2, 43.0, 43.2, 0.0, 10.0, 5, 5, 4)
+    n = g.count
+    nothing = SimpleNamespace(handle=None)
+    models = dict(sP=np.ones(n), sS=np.ones(n), sRefP=np.ones(n), sRefS=np.ones(n), kRef=np.ones(n))
+    with pytest.raises(_lib.HipError) as e:
+        Sy.build_system(nothing, g, Sy.SystemSettings(), nothing, [0], [0], [0], [0], [0.0], [0.1],
+                        np.zeros(1, np.uint8), np.zeros(1, np.uint8), 1, **models, damping=1.0, smoothing=1.0)
+    assert e.value.status == _lib.TS_EINVAL
+    L = Sy._declare()
+    assert L.ts_matrix_shape(None, None, None, None, None) == _lib.TS_EINVAL
+    assert L.ts_matrix_read(None, None, None, None, None) == _lib.TS_EINVAL
+    assert L.ts_matrix_read_transpose(None, None, None, None) == _lib.TS_EINVAL

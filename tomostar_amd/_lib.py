@@ -66,6 +66,7 @@ def load() -> C.CDLL:
     _sig(L, "ts_tables_read_f32", _i, _vp, _i, _i, _vp)
     _sig(L, "ts_tables_rounds", _i, _vp, _vp)
     _sig(L, "ts_tables_stats", _i, _vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d))
+    _sig(L, "ts_tables_sweep_stats", _i, _vp, _vp, _vp, _vp)
     _sig(L, "ts_tables_release", None, _vp)
     for name, res, args in _EXTRA:
         if hasattr(L, name):

@@ -441,6 +441,8 @@ struct ts_tableset {
     bool have64 = false, solved = false;
     std::vector<int32_t> rounds;          // 2 per field
     double evaluations = 0, evaluations32 = 0, evaluations64 = 0, seconds = 0, sweep_seconds = 0;
+    double sweep_by_width[2] = {0, 0};  // sweep-graph device seconds on FP32 / FP64 times
+    int32_t rounds_by_width[2] = {0, 0};  // rounds run on FP32 / FP64 times
     // fast-form working storage, kept across solves
     dbuf<float> T32;
     dbuf<double> T64;
@@ -680,6 +682,8 @@ void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel
         float ms = 0;
         TS_HIP(hipEventElapsedTime(&ms, ev0, ev1));
         t.sweep_seconds += ms * 1e-3;
+        t.sweep_by_width[sizeof(R) == 8] += ms * 1e-3;
+        t.rounds_by_width[sizeof(R) == 8]++;
         if (h_status[1] > 0) fail(TS_DECLINED, "sweeping not settled after %d rounds", sweep_round_cap(t.grid));
         if (h_status[0] == 0) return;
     }
@@ -751,6 +755,8 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     TS_HIP(hipHostMalloc((void **)&h_status, 2 * sizeof(int32_t), hipHostMallocDefault));
     struct Free { int32_t *p; ~Free() { if (p) (void)hipHostFree(p); } } free_h{h_status};
     t.sweep_seconds = 0;
+    t.sweep_by_width[0] = t.sweep_by_width[1] = 0;
+    t.rounds_by_width[0] = t.rounds_by_width[1] = 0;
     if (mixed) {
         run_rounds<float>(t, t.T32.p, count, cancel, h_status);
         eik_mixed_widen<<<blocks_for(nodes_total * kLanes, 256), 256, 0, st>>>(t.T32.p, t.T64.p, nodes_total * kLanes);
@@ -1010,6 +1016,15 @@ int ts_tables_stats(ts_tableset *t, double *evaluations_fp32, double *evaluation
     if (evaluations_fp64) *evaluations_fp64 = t->evaluations64;
     if (seconds) *seconds = t->seconds;
     if (sweep_seconds) *sweep_seconds = t->sweep_seconds;
+    return TS_OK;
+}
+
+int ts_tables_sweep_stats(ts_tableset *t, int32_t rounds[2], int32_t *plane_launches_per_round, double seconds[2])
+{
+    if (!t) return TS_EINVAL;
+    if (rounds) { rounds[0] = t->rounds_by_width[0]; rounds[1] = t->rounds_by_width[1]; }
+    if (plane_launches_per_round) *plane_launches_per_round = 8 * (t->grid.nx + t->grid.ny + t->grid.nz - 2);
+    if (seconds) { seconds[0] = t->sweep_by_width[0]; seconds[1] = t->sweep_by_width[1]; }
     return TS_OK;
 }
 

@@ -19,20 +19,12 @@
 #include <random>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "common.hpp"
+#include "matrix.hpp"
 
 using namespace ts;
-
-struct ts_matrix {
-    ts_context *ctx = nullptr;
-    int rows = 0, cols = 0;
-    int64_t nnz = 0;
-    dbuf<int32_t> rs, ci, trs, tci;  // A and its transpose, CSR, int32 (CsrMatrix's own index type)
-    dbuf<float> va32, tva32;
-    dbuf<double> va64, tva64;        // kept only when the plain FP64 form is enabled
-    double last_seconds = 0;         // device time of the last fast solve's iterations (HIP events)
-    std::mutex gate;
-};
 
 namespace ts {
 
@@ -46,6 +38,96 @@ struct LsqrScal {
 };
 
 constexpr int kSpmmBlocks = 2048;
+
+// ------------------------------------------------------------------ matrix upload and transpose
+
+__global__ void mx_row_of(const int32_t *__restrict__ rs, int rows, int32_t *__restrict__ row_of)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x)
+        for (int32_t q = rs[i]; q < rs[i + 1]; q++) row_of[q] = i;
+}
+
+__global__ void mx_iota(int32_t *__restrict__ v, int64_t n)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        v[q] = (int32_t)q;
+}
+
+__global__ void mx_fill_i32(int32_t *__restrict__ v, int64_t n, int32_t x)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        v[q] = x;
+}
+
+__global__ void mx_count_cols(const int32_t *__restrict__ ci, int64_t nnz, int32_t *__restrict__ count)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&count[ci[q]], 1);
+}
+
+__global__ void mx_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        b[q] = (float)a[q];
+}
+
+// transposed entry p is entry perm[p] of A: its row becomes the column index
+__global__ void mx_gather(const int32_t *__restrict__ perm, const int32_t *__restrict__ row_of,
+                          const double *__restrict__ va, int64_t nnz, int32_t *__restrict__ tci,
+                          float *__restrict__ tva32, double *__restrict__ tva64)
+{
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t q = perm[p];
+        tci[p] = row_of[q];
+        const double v = va[q];
+        tva32[p] = (float)v;
+        if (tva64) tva64[p] = v;
+    }
+}
+
+static unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384)); }
+
+void matrix_finish(ts_matrix &A, dbuf<double> &&d_va64, bool keep_fp64)
+{
+    hipStream_t st = A.ctx->stream;
+    const int64_t nnz = A.nnz;
+    A.trs.alloc((size_t)A.cols + 1);
+    A.tci.alloc(std::max<int64_t>(1, nnz));
+    A.va32.alloc(std::max<int64_t>(1, nnz));
+    A.tva32.alloc(std::max<int64_t>(1, nnz));
+    if (keep_fp64 && nnz > 0) A.tva64.alloc(nnz);
+    // column counts -> transposed row starts
+    {
+        dbuf<int32_t> count((size_t)A.cols + 1);
+        mx_fill_i32<<<grid_for(A.cols + 1), 256, 0, st>>>(count.p, A.cols + 1, 0);
+        if (nnz > 0) mx_count_cols<<<grid_for(nnz), 256, 0, st>>>(A.ci.p, nnz, count.p);
+        size_t tb = 0;
+        TS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, count.p, A.trs.p, A.cols + 1, st));
+        dbuf<unsigned char> tmp(std::max<size_t>(1, tb));
+        TS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, count.p, A.trs.p, A.cols + 1, st));
+        TS_HIP(hipGetLastError());
+        TS_HIP(hipStreamSynchronize(st));
+    }
+    if (nnz > 0) {
+        mx_to_f32<<<grid_for(nnz), 256, 0, st>>>(d_va64.p, A.va32.p, nnz);
+        dbuf<int32_t> row_of(nnz), idx(nnz), perm(nnz), keys(nnz);
+        mx_row_of<<<grid_for(A.rows), 256, 0, st>>>(A.rs.p, A.rows, row_of.p);
+        mx_iota<<<grid_for(nnz), 256, 0, st>>>(idx.p, nnz);
+        int end_bit = 1;
+        while (end_bit < 31 && (1ll << end_bit) < (long long)A.cols) end_bit++;
+        // LSD radix sort is stable: within a column the entries keep their row order
+        size_t tb = 0;
+        TS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, A.ci.p, keys.p, idx.p, perm.p, (int)nnz, 0, end_bit, st));
+        dbuf<unsigned char> tmp(std::max<size_t>(1, tb));
+        TS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, A.ci.p, keys.p, idx.p, perm.p, (int)nnz, 0, end_bit, st));
+        mx_gather<<<grid_for(nnz), 256, 0, st>>>(perm.p, row_of.p, d_va64.p, nnz, A.tci.p, A.tva32.p,
+                                                 keep_fp64 ? A.tva64.p : nullptr);
+        TS_HIP(hipGetLastError());
+        TS_HIP(hipStreamSynchronize(st));
+    }
+    if (keep_fp64 && nnz > 0) A.va64 = std::move(d_va64);
+    else d_va64.reset();
+}
 
 // ------------------------------------------------------------------------------- fast kernels
 
@@ -752,47 +834,19 @@ int ts_matrix_create(ts_context *ctx, int rows, int cols, const int32_t *row_sta
         A->rows = rows;
         A->cols = cols;
         A->nnz = nnz;
-        // the stored transpose: counting sort by column, rows in increasing order (SparseMatrix.cs:79-97)
-        std::vector<int32_t> trs(cols + 1, 0), tci(nnz);
-        std::vector<double> tva(nnz);
-        for (int64_t q = 0; q < nnz; q++) trs[column[q] + 1]++;
-        for (int c = 0; c < cols; c++) trs[c + 1] += trs[c];
-        {
-            std::vector<int32_t> nx(trs.begin(), trs.end() - 1);
-            for (int i = 0; i < rows; i++)
-                for (int32_t q = row_start[i]; q < row_start[i + 1]; q++) {
-                    int32_t p = nx[column[q]]++;
-                    tci[p] = i;
-                    tva[p] = values[q];
-                }
-        }
+        if (nnz >= INT32_MAX) fail(TS_DECLINED, "%lld non-zeros exceed the int32 index range", (long long)nnz);
+        auto t0 = std::chrono::steady_clock::now();
         hipStream_t st = ctx->stream;
         A->rs.alloc(rows + 1);
-        A->trs.alloc(cols + 1);
         A->ci.alloc(std::max<int64_t>(1, nnz));
-        A->tci.alloc(std::max<int64_t>(1, nnz));
-        A->va32.alloc(std::max<int64_t>(1, nnz));
-        A->tva32.alloc(std::max<int64_t>(1, nnz));
+        dbuf<double> va(std::max<int64_t>(1, nnz));
         TS_HIP(hipMemcpyAsync(A->rs.p, row_start, (size_t)(rows + 1) * 4, hipMemcpyHostToDevice, st));
-        TS_HIP(hipMemcpyAsync(A->trs.p, trs.data(), (size_t)(cols + 1) * 4, hipMemcpyHostToDevice, st));
         if (nnz > 0) {
             TS_HIP(hipMemcpyAsync(A->ci.p, column, (size_t)nnz * 4, hipMemcpyHostToDevice, st));
-            TS_HIP(hipMemcpyAsync(A->tci.p, tci.data(), (size_t)nnz * 4, hipMemcpyHostToDevice, st));
-            std::vector<float> f(nnz);
-            for (int64_t q = 0; q < nnz; q++) f[q] = (float)values[q];
-            TS_HIP(hipMemcpyAsync(A->va32.p, f.data(), (size_t)nnz * 4, hipMemcpyHostToDevice, st));
-            TS_HIP(hipStreamSynchronize(st));
-            for (int64_t q = 0; q < nnz; q++) f[q] = (float)tva[q];
-            TS_HIP(hipMemcpyAsync(A->tva32.p, f.data(), (size_t)nnz * 4, hipMemcpyHostToDevice, st));
-            TS_HIP(hipStreamSynchronize(st));
-            if (keep_fp64) {
-                A->va64.alloc(nnz);
-                A->tva64.alloc(nnz);
-                TS_HIP(hipMemcpyAsync(A->va64.p, values, (size_t)nnz * 8, hipMemcpyHostToDevice, st));
-                TS_HIP(hipMemcpyAsync(A->tva64.p, tva.data(), (size_t)nnz * 8, hipMemcpyHostToDevice, st));
-            }
+            TS_HIP(hipMemcpyAsync(va.p, values, (size_t)nnz * 8, hipMemcpyHostToDevice, st));
         }
-        TS_HIP(hipStreamSynchronize(st));
+        matrix_finish(*A, std::move(va), keep_fp64 != 0);
+        A->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         *out = A.release();
         return TS_OK;
     });

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "rays.hpp"
 #include "geometry.hpp"
 
 struct ts_tableset;
@@ -35,17 +36,6 @@ ts_context *tableset_ctx(const ts_tableset *t);
 
 using namespace ts;
 
-struct ts_rays {
-    ts_context *ctx = nullptr;
-    int nrays = 0;
-    std::vector<int32_t> counts;      // nodes per row, -1: no row (stalled or unreachable)
-    std::vector<int64_t> row_start;   // CSR over the arrivals that have rows (empty rows for the others)
-    dbuf<int32_t> nodes;              // device CSR of the rows
-    dbuf<float> lengths;
-    std::vector<double> tcalc_grad;   // 4 per arrival: TCalc, Ge, Gn, Gz
-    std::vector<int32_t> path_points; // points of each path (diagnostic)
-    double seconds = 0;
-};
 
 namespace ts {
 
@@ -360,6 +350,7 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
         std::unique_ptr<ts_rays> R(new ts_rays);
         R->ctx = ctx;
         R->nrays = nrays;
+        R->inversion = *inversion;
         R->counts.assign(nrays, -1);
         R->tcalc_grad.assign(4 * (size_t)nrays, 0.0);
         R->path_points.assign(nrays, 0);

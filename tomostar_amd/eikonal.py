@@ -160,8 +160,14 @@ class TableSet:
     def stats(self):
         """dict: node updates on FP32 and FP64 times, last solve wall seconds and sweep-graph device seconds."""
         a, b, s, k = C.c_double(), C.c_double(), C.c_double(), C.c_double()
-        check(_lib.load().ts_tables_stats(self._h, C.byref(a), C.byref(b), C.byref(s), C.byref(k)))
-        return {"updates_fp32": a.value, "updates_fp64": b.value, "seconds": s.value, "sweep_seconds": k.value}
+        L = _lib.load()
+        check(L.ts_tables_stats(self._h, C.byref(a), C.byref(b), C.byref(s), C.byref(k)))
+        rounds, per_round, secs = np.zeros(2, np.int32), C.c_int32(), np.zeros(2)
+        check(L.ts_tables_sweep_stats(self._h, _lib.ptr(rounds), C.byref(per_round), _lib.ptr(secs)))
+        return {"updates_fp32": a.value, "updates_fp64": b.value, "seconds": s.value, "sweep_seconds": k.value,
+                "rounds_fp32": int(rounds[0]), "rounds_fp64": int(rounds[1]),
+                "plane_launches_per_round": per_round.value,
+                "sweep_seconds_fp32": float(secs[0]), "sweep_seconds_fp64": float(secs[1])}
 
     def release(self):
         if self._h is not None:

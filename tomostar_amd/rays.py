@@ -56,7 +56,7 @@ class RayBatch:
     """Rows of many arrivals traced on the device; arrays in arrival order."""
 
     def __init__(self, tables: TableSet, inversion: GridDefinition, field_station: np.ndarray,
-                 field_of_ray: np.ndarray, events: np.ndarray, step_km: float | None = None):
+                 field_of_ray: np.ndarray, events: np.ndarray, step_km: float | None = None, host_rows: bool = True):
         L = _declare()
         fs = np.ascontiguousarray(field_station, np.float64).reshape(-1, 3)
         if fs.shape[0] != tables.nfields:
@@ -80,16 +80,22 @@ class RayBatch:
         self.row_start = np.zeros(n + 1, np.int64)
         check(L.ts_rays_rows(h, ptr(self.counts), ptr(self.row_start), None, None, None, None))
         nnz = int(self.row_start[-1])
-        self.nodes = np.zeros(max(1, nnz), np.int32)
-        self.lengths = np.zeros(max(1, nnz), np.float32)
+        self.nnz = nnz
+        # the rows stay in HBM for the system assembly; host copies only when asked for
+        self.nodes = np.zeros(max(1, nnz) if host_rows else 1, np.int32)
+        self.lengths = np.zeros(max(1, nnz) if host_rows else 1, np.float32)
         self.tcalc_grad = np.zeros((n, 4))
         self.path_points = np.zeros(n, np.int32)
-        check(L.ts_rays_rows(h, None, None, ptr(self.nodes), ptr(self.lengths), ptr(self.tcalc_grad),
-                             ptr(self.path_points)))
+        check(L.ts_rays_rows(h, None, None, ptr(self.nodes) if host_rows else None,
+                             ptr(self.lengths) if host_rows else None, ptr(self.tcalc_grad), ptr(self.path_points)))
         self.nodes, self.lengths = self.nodes[:nnz], self.lengths[:nnz]
         e, s = C.c_int64(), C.c_double()
         check(L.ts_rays_stats(h, C.byref(e), C.byref(s)))
         self.seconds = s.value
+
+    @property
+    def handle(self):
+        return self._h
 
     def row(self, r: int) -> RayRow | None:
         if self.counts[r] < 0:
