@@ -66,6 +66,171 @@ def cpu_baseline(wl, seeds, mof, budget_s: float = 20.0):
                       f"one field per core, {wall:.1f} s"}
 
 
+def lsqr_bytes_per_iteration(m: int, n: int, nnz: int, k: int, s: int = 4) -> float:
+    """Algorithmic HBM bytes of one LSQR iteration over A and its stored transpose for k right-hand sides:
+    both matrices streamed once (FP32 value + int32 index per non-zero, int32 row starts), and per RHS
+    u read + written, the gathered v and u, v, w, x read + written: 2(8 nnz + 4(m+1)) + k s (3m + 7n)."""
+    return 2 * (8 * nnz + 4 * (m + 1)) + k * s * (3 * m + 7 * n)
+
+
+def lsqr_leg(dev, wl, ts, args):
+    """LSQR at scale: G from curved rays through the batched-eikonal model (20,000 events, the 200
+    stations, first `lsqr_rays` station x event x phase triples of a seeded shuffle), a first-difference
+    smoothing block, k right-hand sides (data residual + independent noise) with damping log-spaced
+    0.1 - 100, a fixed iteration budget. The fast multi-RHS form against the plain FP64 form."""
+    from tomostar_amd import lsqr as Q
+    from tomostar_amd import rays as Ry
+    from tomostar_amd import system as Sy
+    from tomostar_amd import workloads as W
+
+    ev, st_idx, ev_idx, ph = W.scale_arrivals(wl, events=20000, rays=args.lsqr_rays, seed=2)
+    ns, ne = len(wl.stations), len(ev)
+    t0 = time.perf_counter()
+    batch = Ry.RayBatch(ts, wl.grid, np.repeat(wl.stations, 2, axis=0), 2 * st_idx + ph, ev[ev_idx], host_rows=False)
+    t_rays = time.perf_counter() - t0
+    used = np.flatnonzero(batch.counts >= 0).astype(np.int32)
+    log(f"lsqr leg: {args.lsqr_rays} arrivals traced in {t_rays:.2f} s, {used.size} rows, {batch.nnz} non-zeros")
+    rng = np.random.default_rng(3)
+    sigma = 0.05
+    residual = sigma * rng.standard_normal(used.size)
+    sP, sS = wl.slowness_p, wl.slowness_s
+    settings = Sy.SystemSettings(vpvs=False, smoothing_method="gradient", damping_hypocentre=0.0, damping_station=0.0)
+    t0 = time.perf_counter()
+    sysd = Sy.build_system(dev, wl.grid, settings, batch, used, ph[used], ev_idx[used], st_idx[used], residual,
+                           np.full(used.size, sigma), np.zeros(ne, np.uint8), np.zeros(ne, np.uint8), ns, sP, sS,
+                           sP, sS, sS / sP, damping=0.0, smoothing=1.0, keep_fp64=True)
+    t_sys = time.perf_counter() - t0
+    batch.release()
+    A = sysd.matrix
+    m, n, nnz = A.rows, A.cols, A.nnz
+    log(f"lsqr leg: system {m} x {n}, {nnz} non-zeros, assembled in {t_sys:.2f} s")
+    k = args.lsqr_rhs
+    dr = sysd.layout["data_rows"]
+    B = np.repeat(sysd.rhs[None, :], k, axis=0)
+    B[:, :dr] += rng.standard_normal((k, dr))  # N(0, sigma) noise in seconds is N(0, 1) on weighted rows
+    damps = 10.0 ** (-1 + 3 * np.arange(k) / max(1, k - 1))
+    its = args.lsqr_iterations
+    runs = {}
+    for mode in ("fast_mixed", "fast_fp32"):
+        t0 = time.perf_counter()
+        rc, why, res = Q.solve_many(A, B, damps, its, 0.0, 0.0, 0.0, mode=mode)
+        wall = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"{mode} LSQR declined: {why}")
+        runs[mode] = (res, wall, A.last_seconds())
+        log(f"lsqr leg: {mode}, {k} RHS, {max(r.iterations for r in res)} iterations in {wall:.2f} s "
+            f"(device {runs[mode][2]:.3f} s)")
+    # plain FP64 on sampled right-hand sides, each for the iterations its mixed run took, and the
+    # spread rounding alone causes in FP64 (b perturbed by 1e-15) on the first of them
+    picks = sorted(set([0, k // 3, (2 * k) // 3, k - 1]))
+    plain_x, plain_s, plain_its = {}, 0.0, 0
+    for j in picks:
+        t0 = time.perf_counter()
+        rc, why, pr = Q.solve_many(A, B[j:j + 1], [damps[j]], runs["fast_mixed"][0][j].iterations, 0.0, 0.0, 0.0,
+                                   mode="plain_fp64")
+        plain_s += time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"plain LSQR declined: {why}")
+        plain_its += pr[0].iterations
+        plain_x[j] = pr[0].solution
+    bp = B[picks[0]] * (1 + 1e-15 * rng.standard_normal(m))
+    rc, why, pp = Q.solve_many(A, bp[None, :], [damps[picks[0]]], runs["fast_mixed"][0][picks[0]].iterations,
+                               0.0, 0.0, 0.0, mode="plain_fp64")
+    x0 = plain_x[picks[0]]
+    spread = float(np.linalg.norm(pp[0].solution - x0) / np.linalg.norm(x0))
+    plain_rate = plain_its / plain_s                       # RHS-iterations per second
+    forms = {}
+    for mode, (res, wall, dev_s) in runs.items():
+        errs = {str(j): float(np.linalg.norm(res[j].solution - plain_x[j]) / max(1e-300, np.linalg.norm(plain_x[j])))
+                for j in picks}
+        it_max = max(r.iterations for r in res)
+        bpi = lsqr_bytes_per_iteration(m, n, nnz, k, 8 if mode == "fast_mixed" else 4)
+        rate = sum(r.iterations for r in res) / wall
+        forms[mode] = {
+            "rhs_iterations_per_s": round(rate, 2), "seconds": round(wall, 3), "device_seconds": round(dev_s, 3),
+            "iterations": [int(r.iterations) for r in res], "stop_codes": sorted(set(int(r.stop_code) for r in res)),
+            "speedup_vs_plain_fp64_gpu": round(rate / plain_rate, 2),
+            "model_rel_l2_vs_fp64": {"max": max(errs.values()), "per_rhs": errs},
+            "roofline": {"bound": "hbm", "bytes_per_iteration": bpi,
+                         "achieved": round(bpi * it_max / dev_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(bpi * it_max / dev_s / 1e9 / HBM_PEAK_GBS, 4)},
+        }
+    main = forms["fast_mixed"]
+    out = {
+        "workload": f"G of {used.size} curved rays ({args.lsqr_rays} arrivals, 20,000 events x {ns} stations x P,S) "
+                    f"+ first differences on {wl.grid.nx}x{wl.grid.ny}x{wl.grid.nz} (P, S), {k} RHS, "
+                    f"damping 0.1-100, {its} iterations",
+        "rows": m, "columns": n, "nnz": nnz,
+        "rays_seconds": round(t_rays, 3), "system_seconds": round(t_sys, 3),
+        "rhs_iterations_per_s": main["rhs_iterations_per_s"], "unit": "RHS-iterations/s",
+        "form": "fast_mixed (FP32 matrix values, FP64 vectors and scalars)",
+        "speedup_vs_plain_fp64_gpu": main["speedup_vs_plain_fp64_gpu"],
+        "model_rel_l2_vs_fp64": main["model_rel_l2_vs_fp64"]["max"],
+        "fp64_rounding_spread": {"value": spread, "rhs": picks[0],
+                                 "what": "plain FP64 iterate moved by a 1e-15 relative perturbation of b"},
+        "plain_fp64_gpu": {"rhs_iterations_per_s": round(plain_rate, 2), "sample": f"RHS {picks}"},
+        "forms": forms,
+        "roofline": main["roofline"],
+    }
+    errs = [main["model_rel_l2_vs_fp64"]["max"]]
+    out["rhs_iterations_per_s"] = main["rhs_iterations_per_s"]
+    log(f"lsqr leg: {out['rhs_iterations_per_s']} RHS-it/s, {out['speedup_vs_plain_fp64_gpu']}x plain, "
+        f"max model error {max(errs):.2e} (FP64 spread {spread:.2e}), {out['roofline']['achieved']} GB/s")
+    A.release()
+    return out
+
+
+def iteration_leg(dev, cpu: bool):
+    """One tomographic iteration on the 61 x 61 x 31 grid (25 stations x P,S, 150 events): tables,
+    rays, system, 100 LSQR iterations, each stage timed on the GPU and, as the CPU baseline, with the
+    oracle's restatement of the reference's CPU path on the host cores."""
+    from tomostar_amd import iteration as It
+    from tomostar_amd import workloads as W
+
+    wl = W.one_iteration_case(seed=0)
+    t_obs = It.observed_times(dev, wl)
+    It.run(dev, wl, t_obs)  # warm-up: code objects and graphs
+    r = It.run(dev, wl, t_obs)
+    out = {"workload": "61x61x31 at 1 km, 25 stations x P,S, 150 events, Vp/Vs parameterisation, damping 1, "
+                       "Laplacian 5, 100 LSQR iterations",
+           "rows": r.rows, "columns": r.columns, "nnz": r.nnz, "data_rows": r.data_rows,
+           "gpu_seconds": {k: round(v, 4) for k, v in r.stage_seconds.items()}}
+    out["gpu_seconds"]["total"] = round(sum(r.stage_seconds.values()), 4)
+    if cpu:
+        out["cpu_seconds"] = cpu_baseline_iteration(wl, r)
+    log(f"iteration leg: {out}")
+    return out
+
+
+def cpu_baseline_iteration(wl, r):
+    """The same iteration on the oracle (the reference's CPU solvers restated: fast marching one table
+    per core, rays in parallel over arrivals, the system, LSQR with 16-thread SpMV)."""
+    from oracle import oracle as O
+    from tomostar_amd import iteration as It
+
+    g = wl.grid
+    og = O.griddef(g.min_lon, g.max_lon, g.min_lat, g.max_lat, g.min_depth, g.max_depth, g.nx, g.ny, g.nz)
+    cores = min(os.cpu_count() or 1, 16)
+    sP, sS = It.starting_model(wl)
+    sec = {}
+    t0 = time.perf_counter()
+    seeds = [O.point_source(og, sP if ph == 0 else sS, *st) for st in wl.stations for ph in (0, 1)]
+    tabs = O.fmm_batch(og, [sP if q % 2 == 0 else sS for q in range(len(seeds))], seeds, threads=cores)
+    sec["tables"] = time.perf_counter() - t0
+    station, event, phase = r.arrivals
+    t0 = time.perf_counter()
+    O.ray_rows_batch(og, [t.astype(np.float32) for t in tabs], np.repeat(wl.stations, 2, axis=0), og,
+                     2 * station + phase, wl.events[event], 0.5 * O.min_spacing_km(og), threads=cores)
+    sec["rays"] = time.perf_counter() - t0
+    rs, ci, va = r.system.read()
+    t0 = time.perf_counter()
+    O.lsqr(r.rows, r.columns, rs, ci, va, r.rhs, damp=0.0, max_it=100, atol=0, btol=0, conlim=0, threads=cores)
+    sec["lsqr"] = time.perf_counter() - t0
+    sec = {k: round(v, 4) for k, v in sec.items()}
+    sec["cores"] = cores
+    return sec
+
+
 def calibrate_copy(mib: int) -> int:
     """A plain device-to-device copy of known size: the reference launch that hardware byte counters
     (FETCH_SIZE, WRITE_SIZE) are scaled by."""
@@ -90,6 +255,10 @@ def main() -> int:
     ap.add_argument("--stations", type=int, default=200)
     ap.add_argument("--plain-fields", type=int, default=23, help="fields of the plain-FP64 sample (one batch); 0 skips it")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lsqr-rays", type=int, default=1_000_000, help="arrivals of the LSQR-at-scale leg (0 skips it)")
+    ap.add_argument("--lsqr-rhs", type=int, default=32)
+    ap.add_argument("--lsqr-iterations", type=int, default=200)
+    ap.add_argument("--no-iteration", action="store_true", help="skip the one-iteration pipeline leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (counter calibration) and exit")
     args = ap.parse_args()
@@ -179,6 +348,10 @@ def main() -> int:
         cpu = cpu_baseline(wl, seeds, mof)
         log(f"cpu baseline: {cpu}")
 
+    lsqr = lsqr_leg(dev, wl, ts, args) if args.lsqr_rays > 0 else None
+    ts.release()
+    it_leg = None if args.no_iteration else iteration_leg(dev, cpu=not args.no_cpu)
+
     out = {
         "metric": "eikonal Gnode-upd/s + LSQR iters/s vs plain FP64 GPU; p50/p99 solve latency",
         "value": round(value, 4),
@@ -209,9 +382,10 @@ def main() -> int:
                      "kernels": per_kernel,
                      "sweep_seconds_per_step": float(np.mean(sweep))},
         "cpu_baseline": cpu,
+        "lsqr": lsqr,
+        "iteration": it_leg,
     }
     print(json.dumps(out), flush=True)
-    ts.release()
     dev.close()
     return 0
 

@@ -52,6 +52,9 @@ extern "C" {
                                 side at a time, host-side rotations, three blocking reads an iteration */
 #define TS_LSQR_FAST_FP32 1  /* k right-hand sides share each pass over A and A^T (SpMM); FP32 values and
                                 u/v/w, FP64 row sums, x and scalars, all on the device, graph per iteration */
+#define TS_LSQR_FAST_MIXED 2 /* as TS_LSQR_FAST_FP32 with FP64 u/v/w: only the matrix values are FP32.
+                                FP64 operands keep the iterates on the FP64 solver's path; FP32 operands
+                                of an ill-conditioned system drift by ~1e-3 within tens of iterations */
 
 typedef struct ts_context ts_context;
 typedef struct ts_tableset ts_tableset;

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libtomostar_hip.so")
 TS_OK, TS_DECLINED, TS_EINVAL, TS_CANCELLED, TS_EDEVICE = 0, 1, -1, -2, -3
 TS_CREATE_SKIP_SELFTEST = 1
 TS_EIK_PLAIN_FP64, TS_EIK_FAST_FP32, TS_EIK_FAST_FP64 = 0, 1, 2
-TS_LSQR_PLAIN_FP64, TS_LSQR_FAST_FP32 = 0, 1
+TS_LSQR_PLAIN_FP64, TS_LSQR_FAST_FP32, TS_LSQR_FAST_MIXED = 0, 1, 2
 
 
 class TsGrid(C.Structure):

@@ -1,18 +1,23 @@
 // LSQR on gfx950: damped LSQR (Paige & Saunders; the SOL form of Numerics/Lsqr.cs:34-152) with A and
 // its stored transpose resident in HBM as CSR (Numerics/SparseMatrix.cs:24-97).
 //
-// Two device forms:
+// Device forms:
 //   plain (TS_LSQR_PLAIN_FP64): the reference's device loop (Numerics/LsqrOpenCl.cs:272-533) on this GPU -
 //     FP64 values and vectors, one right-hand side at a time, a row-per-thread CSR product, one launch
 //     per vector operation, dot products as 64 fixed partial sums added on the host in order, and the
 //     plane rotations on the host (three blocking reads per iteration).
-//   fast (TS_LSQR_FAST_FP32): k right-hand sides with their own damping share every pass over A and
-//     A^T (SpMM over [row][k]-interleaved vectors: one gather of k contiguous values per non-zero).
-//     Matrix values and the u, v, w vectors are FP32; row sums, the solution x and every scalar (the
-//     norms alpha and beta, the renormalisation factors, the rotations and the stopping estimates)
-//     are FP64 on the device. u and v are stored unnormalised with an FP64 scale each, so a
-//     renormalisation costs no pass. An iteration is five kernels captured once in a graph; the host
-//     only looks at the stop flags every few iterations.
+//   fast (TS_LSQR_FAST_FP32, TS_LSQR_FAST_MIXED): k right-hand sides with their own damping share every
+//     pass over A and A^T (SpMM over [row][k]-interleaved vectors: one gather of k contiguous values
+//     per non-zero). Matrix values are FP32; the u, v, w vectors are FP32 (FAST_FP32) or FP64
+//     (FAST_MIXED); row sums, the solution x and every scalar (the norms alpha and beta, the
+//     renormalisation factors, the rotations and the stopping estimates) are FP64 on the device. u
+//     and v are stored unnormalised with an FP64 scale each, so a renormalisation costs no pass. An
+//     iteration is five kernels captured once in a graph; the host only looks at the stop flags every
+//     few iterations.
+//     Why two vector widths: u' = A v - alpha u cancels heavily on ill-conditioned systems, so the
+//     rounding of the gathered operand v (not of A) is amplified; FP32 operands move the iterates by
+//     ~1e-3 within 20 iterations on a regularised tomography system where FP64 operands agree with the
+//     FP64 oracle to 1e-8 (FP32 matrix values alone perturb the problem consistently and cost little).
 #include <algorithm>
 #include <chrono>
 #include <memory>
@@ -133,10 +138,10 @@ void matrix_finish(ts_matrix &A, dbuf<double> &&d_va64, bool keep_fp64)
 
 // Y[r][j] = sc[j] * sum_q va[q] X[ci[q]][j] - sc[KP + j] * Y[r][j] for all rows, FP64 row sums; the
 // squares of the stored results are summed per lane and reduced per block in a fixed order.
-template <int KP>
+template <int KP, typename T>
 __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__restrict__ rs,
                                                  const int32_t *__restrict__ ci, const float *__restrict__ va,
-                                                 const float *__restrict__ X, float *__restrict__ Y,
+                                                 const T *__restrict__ X, T *__restrict__ Y,
                                                  const double *__restrict__ sc, double *__restrict__ partial)
 {
     constexpr int G = 256 / KP;
@@ -150,8 +155,8 @@ __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__rest
         for (; q + 4 <= b; q += 4) {
             const int c0 = ci[q], c1 = ci[q + 1], c2 = ci[q + 2], c3 = ci[q + 3];
             const float v0 = va[q], v1 = va[q + 1], v2 = va[q + 2], v3 = va[q + 3];
-            const float x0 = X[(size_t)c0 * KP + j], x1 = X[(size_t)c1 * KP + j];
-            const float x2 = X[(size_t)c2 * KP + j], x3 = X[(size_t)c3 * KP + j];
+            const T x0 = X[(size_t)c0 * KP + j], x1 = X[(size_t)c1 * KP + j];
+            const T x2 = X[(size_t)c2 * KP + j], x3 = X[(size_t)c3 * KP + j];
             acc0 += (double)v0 * (double)x0;
             acc1 += (double)v1 * (double)x1;
             acc0 += (double)v2 * (double)x2;
@@ -160,7 +165,7 @@ __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__rest
         for (; q < b; q++) acc0 += (double)va[q] * (double)X[(size_t)ci[q] * KP + j];
         const size_t at = (size_t)r * KP + j;
         const double old = cy != 0.0 ? cy * (double)Y[at] : 0.0;
-        const float y = (float)(sx * (acc0 + acc1) - old);
+        const T y = (T)(sx * (acc0 + acc1) - old);
         Y[at] = y;
         sq += (double)y * (double)y;
     }
@@ -175,8 +180,8 @@ __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__rest
 }
 
 // x += cx w ; w = cv V + cw w over n rows; squares of the new w per block (for the next ddnorm).
-template <int KP>
-__global__ __launch_bounds__(256) void lsqr_xw(int n, const float *__restrict__ V, float *__restrict__ W,
+template <int KP, typename T>
+__global__ __launch_bounds__(256) void lsqr_xw(int n, const T *__restrict__ V, T *__restrict__ W,
                                                double *__restrict__ Xs, const LsqrScal *__restrict__ S,
                                                double *__restrict__ partial)
 {
@@ -188,7 +193,7 @@ __global__ __launch_bounds__(256) void lsqr_xw(int n, const float *__restrict__ 
         const size_t at = (size_t)r * KP + j;
         const double w = W[at];
         Xs[at] += cx * w;
-        const float wn = (float)(cv * (double)V[at] + cw * w);
+        const T wn = (T)(cv * (double)V[at] + cw * w);
         W[at] = wn;
         sq += (double)wn * (double)wn;
     }
@@ -340,13 +345,14 @@ __global__ void lsqr_fin_init(int k, int kp, LsqrScal *S, const double *partial_
     S[j] = s;
 }
 
-__global__ void lsqr_load_b(int m, int k, int kp, const double *b, float *U, const LsqrScal *S)
+template <typename T>
+__global__ void lsqr_load_b(int m, int k, int kp, const double *b, T *U, const LsqrScal *S)
 {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (size_t)m * kp) return;
     const int j = (int)(q % kp);
     const size_t r = q / kp;
-    U[q] = j < k ? (float)b[(size_t)j * m + r] : 0.0f;
+    U[q] = j < k ? (T)b[(size_t)j * m + r] : (T)0;
 }
 
 __global__ void lsqr_unpack_x(int n, int k, int kp, const double *Xs, double *x)
@@ -534,31 +540,32 @@ void lsqr_plain(ts_matrix &A, int k, const double *b, const double *damp, int ma
 }
 
 // The fast form's device state, kept on the matrix handle's side between the create and the solve.
+template <typename T>
 struct FastLsqr {
     int k = 0, kp = 0, max_it = 0;
-    dbuf<float> U, V, W;
+    dbuf<T> U, V, W;
     dbuf<double> X, pu, pv, pw, sc_a, sc_t, hist, bd;
     dbuf<LsqrScal> S;
 };
 
-template <int KP>
-void fast_iteration(ts_matrix &A, FastLsqr &F, hipStream_t st, bool with_hist)
+template <int KP, typename T>
+void fast_iteration(ts_matrix &A, FastLsqr<T> &F, hipStream_t st, bool with_hist)
 {
-    lsqr_spmm<KP><<<kSpmmBlocks, 256, 0, st>>>(A.rows, A.rs.p, A.ci.p, A.va32.p, F.V.p, F.U.p, F.sc_a.p, F.pu.p);
+    lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.rows, A.rs.p, A.ci.p, A.va32.p, F.V.p, F.U.p, F.sc_a.p, F.pu.p);
     lsqr_fin_u<<<1, 64, 0, st>>>(F.k, KP, F.S.p, F.pu.p, kSpmmBlocks, F.sc_t.p);
-    lsqr_spmm<KP><<<kSpmmBlocks, 256, 0, st>>>(A.cols, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
+    lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.cols, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
     lsqr_fin_v<<<1, 64, 0, st>>>(F.k, KP, F.S.p, F.pv.p, F.pw.p, kSpmmBlocks, kSpmmBlocks, F.sc_a.p,
                                  with_hist ? F.hist.p : nullptr, F.max_it);
-    lsqr_xw<KP><<<kSpmmBlocks, 256, 0, st>>>(A.cols, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
+    lsqr_xw<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.cols, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
 }
 
-template <int KP>
+template <int KP, typename T>
 void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int max_it, double atol, double btol,
                   double conlim, const volatile int32_t *cancel, SolveOut o, double *timing)
 {
     hipStream_t st = A.ctx->stream;
     const int m = A.rows, n = A.cols;
-    FastLsqr F;
+    FastLsqr<T> F;
     F.k = k;
     F.kp = KP;
     F.max_it = max_it;
@@ -602,17 +609,17 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
         s.su = 1.0 / s.beta;
     }
     TS_HIP(hipMemcpyAsync(F.S.p, S.data(), KP * sizeof(LsqrScal), hipMemcpyHostToDevice, st));
-    lsqr_load_b<<<nblk((size_t)m * KP, 256), 256, 0, st>>>(m, k, KP, F.bd.p, F.U.p, F.S.p);
-    TS_HIP(hipMemsetAsync(F.V.p, 0, (size_t)n * KP * 4, st));
-    TS_HIP(hipMemsetAsync(F.W.p, 0, (size_t)n * KP * 4, st));
+    lsqr_load_b<T><<<nblk((size_t)m * KP, 256), 256, 0, st>>>(m, k, KP, F.bd.p, F.U.p, F.S.p);
+    TS_HIP(hipMemsetAsync(F.V.p, 0, (size_t)n * KP * sizeof(T), st));
+    TS_HIP(hipMemsetAsync(F.W.p, 0, (size_t)n * KP * sizeof(T), st));
     TS_HIP(hipMemsetAsync(F.X.p, 0, (size_t)n * KP * 8, st));
     std::vector<double> sc(2 * KP);
     for (int j = 0; j < KP; j++) { sc[j] = S[j].done ? 0.0 : S[j].su; sc[KP + j] = 0; }
     TS_HIP(hipMemcpyAsync(F.sc_t.p, sc.data(), 2 * KP * 8, hipMemcpyHostToDevice, st));
     // v = A^T u, alpha, w = v
-    lsqr_spmm<KP><<<kSpmmBlocks, 256, 0, st>>>(n, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
+    lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(n, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
     lsqr_fin_init<<<1, 64, 0, st>>>(k, KP, F.S.p, F.pv.p, kSpmmBlocks, F.sc_a.p);
-    lsqr_xw<KP><<<kSpmmBlocks, 256, 0, st>>>(n, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
+    lsqr_xw<KP, T><<<kSpmmBlocks, 256, 0, st>>>(n, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
     TS_HIP(hipGetLastError());
     TS_HIP(hipStreamSynchronize(st));
     // one iteration as a graph, replayed; the stop flags are read every `check` iterations
@@ -620,7 +627,7 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     if (A.ctx->use_graphs) {
         hipGraph_t gr;
         TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        fast_iteration<KP>(A, F, st, o.hist != nullptr);
+        fast_iteration<KP, T>(A, F, st, o.hist != nullptr);
         TS_HIP(hipStreamEndCapture(st, &gr));
         hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
@@ -640,7 +647,7 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
         int todo = std::min(check, max_it - it);
         for (int q = 0; q < todo; q++) {
             if (ge) TS_HIP(hipGraphLaunch(ge, st));
-            else fast_iteration<KP>(A, F, st, o.hist != nullptr);
+            else fast_iteration<KP, T>(A, F, st, o.hist != nullptr);
         }
         it += todo;
         TS_HIP(hipMemcpyAsync(cur.data(), F.S.p, KP * sizeof(LsqrScal), hipMemcpyDeviceToHost, st));
@@ -668,16 +675,25 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     }
 }
 
-void lsqr_fast(ts_matrix &A, int k, const double *b, const double *damp, int max_it, double atol, double btol,
-               double conlim, const volatile int32_t *cancel, SolveOut o, double *timing)
+template <typename T>
+void lsqr_fast_t(ts_matrix &A, int k, const double *b, const double *damp, int max_it, double atol, double btol,
+                 double conlim, const volatile int32_t *cancel, SolveOut o, double *timing)
 {
-    if (k <= 1) lsqr_fast_kp<1>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 2) lsqr_fast_kp<2>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 4) lsqr_fast_kp<4>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 8) lsqr_fast_kp<8>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 16) lsqr_fast_kp<16>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 32) lsqr_fast_kp<32>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else lsqr_fast_kp<64>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    if (k <= 1) lsqr_fast_kp<1, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 2) lsqr_fast_kp<2, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 4) lsqr_fast_kp<4, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 8) lsqr_fast_kp<8, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 16) lsqr_fast_kp<16, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 32) lsqr_fast_kp<32, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else lsqr_fast_kp<64, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+}
+
+// vectors in FP32 (TS_LSQR_FAST_FP32) or FP64 (TS_LSQR_FAST_MIXED); matrix values FP32 in both
+void lsqr_fast(ts_matrix &A, int k, const double *b, const double *damp, int max_it, double atol, double btol,
+               double conlim, const volatile int32_t *cancel, SolveOut o, double *timing, bool fp64_vectors)
+{
+    if (fp64_vectors) lsqr_fast_t<double>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else lsqr_fast_t<float>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
 }
 
 // ------------------------------------------------------------------------------- self-test
@@ -873,7 +889,7 @@ int ts_lsqr_solve(ts_matrix *a, int k, const double *b, const double *damp, int 
                   int reason_len)
 {
     if (!a || k < 1 || k > 64 || !b || !damp || max_iterations < 0 || !x || !iterations || !stop || !stats ||
-        (mode != TS_LSQR_PLAIN_FP64 && mode != TS_LSQR_FAST_FP32)) {
+        (mode != TS_LSQR_PLAIN_FP64 && mode != TS_LSQR_FAST_FP32 && mode != TS_LSQR_FAST_MIXED)) {
         set_reason(reason, reason_len, "invalid argument");
         return TS_EINVAL;
     }
@@ -891,7 +907,8 @@ int ts_lsqr_solve(ts_matrix *a, int k, const double *b, const double *damp, int 
         }
         SolveOut o{x, iterations, stop, stats, rnorm_history};
         if (mode == TS_LSQR_PLAIN_FP64) lsqr_plain(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o);
-        else lsqr_fast(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o, &a->last_seconds);
+        else lsqr_fast(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o, &a->last_seconds,
+                       mode == TS_LSQR_FAST_MIXED);
         return TS_OK;
     });
 }

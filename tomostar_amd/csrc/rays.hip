@@ -160,10 +160,13 @@ __global__ __launch_bounds__(64) void ray_trace(Grid g, const float *__restrict_
     npts[job.ray] = status == 0 ? n : status;
 }
 
-constexpr int kHashSlots = 2048;      // per ray: distinct nodes it may touch (load kept under 1/2)
 constexpr int kChunkSeg = 32;         // path segments integrated per pass
 constexpr int kMaxPiecesSeg = 4;      // pieces a segment may split into within one pass buffer
-constexpr int kRowCap = 1024;         // nodes a row may hold
+// Rows are formed in an LDS hash of kSlots entries (load kept under 1/2: a row holds at most
+// kSlots / 2 nodes). Most rays fit the small table; the few longer ones are formed again with the
+// large one.
+constexpr int kSlotsSmall = 2048;
+constexpr int kSlotsLarge = 8192;
 
 struct RowsArgs {
     Grid inv;
@@ -171,15 +174,19 @@ struct RowsArgs {
     const double *path;
     const RayJob *jobs;
     const int32_t *npts;
-    int njobs;
-    int32_t *row_nodes;                // [job][kRowCap]
+    const int32_t *job_of_block;       // the jobs this launch forms (null: block q forms job q)
+    int nblocks;
+    int32_t *row_nodes;                // [job][row_cap]
     float *row_len;
+    int64_t row_cap;
     int32_t *row_count;                // per job: nodes, -1 no row, -3 too many nodes
 };
 
+template <int kHashSlots>
 __device__ inline uint32_t slot_hash(int32_t node) { return ((uint32_t)node * 2654435761u) & (kHashSlots - 1); }
 
 // RayTracing.Kernel (RayTracing.cs:126-159) for one arrival per wavefront.
+template <int kHashSlots>
 __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
 {
     __shared__ int32_t hkey[kHashSlots];
@@ -189,12 +196,13 @@ __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
     __shared__ int32_t pstart[kChunkSeg + 1];
     __shared__ int overflow;
     const int lane = threadIdx.x;
-    const int q = blockIdx.x;
-    if (q >= A.njobs) return;
+    if ((int)blockIdx.x >= A.nblocks) return;
+    const int q = A.job_of_block ? A.job_of_block[blockIdx.x] : (int)blockIdx.x;
+    const int slot = blockIdx.x;  // the row's place in this launch's output
     const RayJob job = A.jobs[q];
     const int np = A.npts[job.ray];
     if (np < 0) {
-        if (lane == 0) A.row_count[q] = -1;
+        if (lane == 0) A.row_count[slot] = -1;
         return;
     }
     for (int s = lane; s < kHashSlots; s += 64) hkey[s] = -1;
@@ -257,7 +265,7 @@ __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
                 if (sub == 0) {
                     const int node = cnode[pi * 8 + c];
                     if (node >= 0) {
-                        uint32_t slot = slot_hash(node);
+                        uint32_t slot = slot_hash<kHashSlots>(node);
                         for (int probe = 0; probe < kHashSlots; probe++) {
                             int k0 = hkey[slot];
                             if (k0 == node) { hval[slot] += cval[pi * 8 + c]; break; }
@@ -296,27 +304,44 @@ __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
     int count = 0;
     for (int i = lane; i < kHashSlots; i += 64) count += hkey[i] != -1;
     for (int off = 32; off > 0; off >>= 1) count += __shfl_xor(count, off);
-    if (count > kRowCap || overflow) {
-        if (lane == 0) A.row_count[q] = -3;
+    if (count > A.row_cap || overflow) {
+        if (lane == 0) A.row_count[slot] = -3;
         return;
     }
     for (int i = lane; i < count; i += 64) {
-        A.row_nodes[(size_t)q * kRowCap + i] = hkey[i];
-        A.row_len[(size_t)q * kRowCap + i] = (float)hval[i];
+        A.row_nodes[(size_t)slot * A.row_cap + i] = hkey[i];
+        A.row_len[(size_t)slot * A.row_cap + i] = (float)hval[i];
     }
-    if (lane == 0) A.row_count[q] = count;
+    if (lane == 0) A.row_count[slot] = count;
 }
 
-__global__ void ray_compact(const int32_t *row_nodes, const float *row_len, const int32_t *row_count,
-                            const int64_t *row_at, int njobs, int32_t *nodes, float *lengths)
+// rows of a batch (each in its row_cap slot) to their places in the arrival-ordered CSR; dst_at < 0:
+// the job has no row to keep
+__global__ void ray_place(const int32_t *row_nodes, const float *row_len, int64_t row_cap, const int32_t *row_count,
+                          const int64_t *dst_at, int njobs, int32_t *nodes, float *lengths)
 {
     const int q = blockIdx.x;
     if (q >= njobs) return;
     const int c = row_count[q];
-    const int64_t at = row_at[q];
+    const int64_t at = dst_at[q];
+    if (at < 0 || c <= 0) return;
     for (int i = threadIdx.x; i < c; i += blockDim.x) {
-        nodes[at + i] = row_nodes[(size_t)q * kRowCap + i];
-        lengths[at + i] = row_len[(size_t)q * kRowCap + i];
+        nodes[at + i] = row_nodes[(size_t)q * row_cap + i];
+        lengths[at + i] = row_len[(size_t)q * row_cap + i];
+    }
+}
+
+// tight rows of a batch to their places in the arrival-ordered CSR
+__global__ void ray_move(const int32_t *src_nodes, const float *src_len, const int64_t *src_at, const int64_t *dst_at,
+                         const int32_t *count, int n, int32_t *nodes, float *lengths)
+{
+    const int q = blockIdx.x;
+    if (q >= n) return;
+    const int c = count[q];
+    const int64_t a = src_at[q], b = dst_at[q];
+    for (int i = threadIdx.x; i < c; i += blockDim.x) {
+        nodes[b + i] = src_nodes[a + i];
+        lengths[b + i] = src_len[a + i];
     }
 }
 
@@ -366,13 +391,87 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
         dbuf<int32_t> d_npts(std::max(1, nrays));
         TS_HIP(hipMemcpyAsync(d_st.p, field_station, 3 * (size_t)nf * 8, hipMemcpyHostToDevice, st));
         if (nrays > 0) TS_HIP(hipMemcpyAsync(d_ev.p, events, 3 * (size_t)nrays * 8, hipMemcpyHostToDevice, st));
-        std::vector<int32_t> counts_sorted;
-        std::vector<std::vector<int32_t>> nodes_parts;
-        std::vector<std::vector<float>> len_parts;
+        // A batch keeps its formed rows on the device until every batch is done; the rows are then
+        // placed in arrival order (RayKernels.cs:114) without passing through the host.
+        struct Formed {
+            dbuf<int32_t> nodes;             // the rows of a launch, back to back
+            dbuf<float> len;
+            std::vector<int32_t> ray, count; // per kept row: arrival and length
+            std::vector<int64_t> at;         // per kept row: offset in nodes / len
+        };
+        std::vector<Formed> formed;
         const int batch = 65536;
         auto t0 = std::chrono::steady_clock::now();
+        // rows of one launch (row_cap slots) packed back to back; keep[q] < 0 drops job q
+        auto pack = [&](const dbuf<int32_t> &rn, const dbuf<float> &rl, int64_t cap, const dbuf<int32_t> &rc,
+                        const std::vector<int32_t> &cnt, const std::vector<int32_t> &ray_of) {
+            const int n = (int)cnt.size();
+            Formed F;
+            std::vector<int64_t> dst(n, -1);
+            int64_t tot = 0;
+            for (int q = 0; q < n; q++)
+                if (ray_of[q] >= 0 && cnt[q] > 0) {
+                    dst[q] = tot;
+                    F.ray.push_back(ray_of[q]);
+                    F.count.push_back(cnt[q]);
+                    F.at.push_back(tot);
+                    tot += cnt[q];
+                }
+            if (tot == 0) return;
+            F.nodes.alloc(tot);
+            F.len.alloc(tot);
+            dbuf<int64_t> d_dst(n);
+            TS_HIP(hipMemcpyAsync(d_dst.p, dst.data(), n * 8, hipMemcpyHostToDevice, st));
+            ray_place<<<n, 256, 0, st>>>(rn.p, rl.p, cap, rc.p, d_dst.p, n, F.nodes.p, F.len.p);
+            TS_HIP(hipGetLastError());
+            TS_HIP(hipStreamSynchronize(st));
+            formed.push_back(std::move(F));
+        };
+        auto form_rows = [&](const dbuf<double> &d_path, const dbuf<RayJob> &d_jobs, const std::vector<RayJob> &jobs,
+                             const std::vector<int32_t> &npts_all) {
+            const int nb = (int)jobs.size();
+            const int64_t cap = kSlotsSmall / 2;
+            dbuf<int32_t> rn((size_t)nb * cap), rc(nb);
+            dbuf<float> rl((size_t)nb * cap);
+            RowsArgs A{inv, h, d_path.p, d_jobs.p, d_npts.p, nullptr, nb, rn.p, rl.p, cap, rc.p};
+            ray_rows<kSlotsSmall><<<nb, 64, 0, st>>>(A);
+            TS_HIP(hipGetLastError());
+            std::vector<int32_t> cnt(nb);
+            TS_HIP(hipMemcpyAsync(cnt.data(), rc.p, nb * 4, hipMemcpyDeviceToHost, st));
+            TS_HIP(hipStreamSynchronize(st));
+            std::vector<int32_t> ray_of(nb), longer;
+            for (int q = 0; q < nb; q++) {
+                const int r = jobs[q].ray;
+                R->path_points[r] = npts_all[r];
+                ray_of[q] = r;
+                if (cnt[q] == -3) longer.push_back(q);
+                else R->counts[r] = cnt[q] >= 0 ? cnt[q] : -1;
+            }
+            pack(rn, rl, cap, rc, cnt, ray_of);
+            // rows too long for the small table: formed again with the large one
+            if (!longer.empty()) {
+                const int nl = (int)longer.size();
+                const int64_t capl = kSlotsLarge / 2;
+                dbuf<int32_t> ln((size_t)nl * capl), lc(nl), which(nl);
+                dbuf<float> ll((size_t)nl * capl);
+                TS_HIP(hipMemcpyAsync(which.p, longer.data(), nl * 4, hipMemcpyHostToDevice, st));
+                RowsArgs B{inv, h, d_path.p, d_jobs.p, d_npts.p, which.p, nl, ln.p, ll.p, capl, lc.p};
+                ray_rows<kSlotsLarge><<<(unsigned)nl, 64, 0, st>>>(B);
+                TS_HIP(hipGetLastError());
+                std::vector<int32_t> cnt2(nl), ray2(nl);
+                TS_HIP(hipMemcpyAsync(cnt2.data(), lc.p, nl * 4, hipMemcpyDeviceToHost, st));
+                TS_HIP(hipStreamSynchronize(st));
+                for (int i = 0; i < nl; i++) {
+                    const int r = jobs[longer[i]].ray;
+                    if (cnt2[i] == -3) fail(TS_DECLINED, "ray %d touches more than %lld nodes", r, (long long)capl);
+                    ray2[i] = r;
+                    R->counts[r] = cnt2[i] >= 0 ? cnt2[i] : -1;
+                }
+                pack(ln, ll, capl, lc, cnt2, ray2);
+            }
+        };
         for (int b0 = 0; b0 < nrays; b0 += batch) {
-            int nb = std::min(batch, nrays - b0);
+            const int nb = std::min(batch, nrays - b0);
             // path slots sized from the straight distance; a path that outgrows its slot is traced again
             std::vector<RayJob> jobs(nb);
             int64_t at = 0;
@@ -385,26 +484,20 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
                 jobs[q] = RayJob{r, f, at, cap};
                 at += cap;
             }
-            for (int attempt = 0; attempt < 2; attempt++) {
+            std::vector<int32_t> npts_all(nrays);
+            std::vector<RayJob> redo;
+            {
                 dbuf<RayJob> d_jobs(nb);
                 dbuf<double> d_path(3 * (size_t)at);
                 TS_HIP(hipMemcpyAsync(d_jobs.p, jobs.data(), nb * sizeof(RayJob), hipMemcpyHostToDevice, st));
                 ray_trace<<<(nb + 63) / 64, 64, 0, st>>>(fwd, tab, d_st.p, d_ev.p, d_jobs.p, nb, step_km, d_path.p,
                                                           d_npts.p, d_tcg.p);
                 TS_HIP(hipGetLastError());
-                dbuf<int32_t> rn((size_t)nb * kRowCap), rc(nb);
-                dbuf<float> rl((size_t)nb * kRowCap);
-                RowsArgs A{inv, h, d_path.p, d_jobs.p, d_npts.p, nb, rn.p, rl.p, rc.p};
-                ray_rows<<<nb, 64, 0, st>>>(A);
-                TS_HIP(hipGetLastError());
-                std::vector<int32_t> cnt(nb), npts_all(nrays);
-                TS_HIP(hipMemcpyAsync(cnt.data(), rc.p, nb * 4, hipMemcpyDeviceToHost, st));
                 TS_HIP(hipMemcpyAsync(npts_all.data(), d_npts.p, (size_t)nrays * 4, hipMemcpyDeviceToHost, st));
                 TS_HIP(hipStreamSynchronize(st));
-                // paths that outgrew their slots: trace those again with the bound of Backtrack's loop
-                std::vector<RayJob> redo;
+                // paths that outgrew their slots are traced again with the bound of Backtrack's loop
                 int64_t at2 = 0;
-                for (int q = 0; q < nb; q++)
+                for (int q = 0; q < nb; q++) {
                     if (npts_all[jobs[q].ray] == -2) {
                         RayJob j = jobs[q];
                         const int r = j.ray, f = j.field;
@@ -415,38 +508,23 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
                         at2 += j.path_cap;
                         redo.push_back(j);
                     }
-                // keep the finished rows
-                std::vector<int64_t> row_at(nb + 1, 0);
-                for (int q = 0; q < nb; q++) row_at[q + 1] = row_at[q] + std::max(0, cnt[q]);
-                dbuf<int64_t> d_row_at(nb + 1);
-                dbuf<int32_t> cn(std::max<int64_t>(1, row_at[nb]));
-                dbuf<float> cl(std::max<int64_t>(1, row_at[nb]));
-                TS_HIP(hipMemcpyAsync(d_row_at.p, row_at.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st));
-                ray_compact<<<nb, 256, 0, st>>>(rn.p, rl.p, rc.p, d_row_at.p, nb, cn.p, cl.p);
-                std::vector<int32_t> hn(row_at[nb]);
-                std::vector<float> hl(row_at[nb]);
-                if (row_at[nb] > 0) {
-                    TS_HIP(hipMemcpyAsync(hn.data(), cn.p, row_at[nb] * 4, hipMemcpyDeviceToHost, st));
-                    TS_HIP(hipMemcpyAsync(hl.data(), cl.p, row_at[nb] * 4, hipMemcpyDeviceToHost, st));
                 }
-                TS_HIP(hipStreamSynchronize(st));
-                for (int q = 0; q < nb; q++) {
-                    const int r = jobs[q].ray;
-                    if (npts_all[r] == -2 && attempt == 0) continue;
-                    R->counts[r] = cnt[q] >= 0 ? cnt[q] : -1;
-                    R->path_points[r] = npts_all[r];
-                    if (cnt[q] == -3) fail(TS_DECLINED, "ray %d touches more than %d nodes", r, kRowCap);
-                    if (cnt[q] > 0) {
-                        nodes_parts.emplace_back(hn.begin() + row_at[q], hn.begin() + row_at[q + 1]);
-                        len_parts.emplace_back(hl.begin() + row_at[q], hl.begin() + row_at[q + 1]);
-                        counts_sorted.push_back(r);
-                    }
+                form_rows(d_path, d_jobs, jobs, npts_all);
+                if (!redo.empty()) {
+                    // their first paths were cut short (no row was formed): the retrace below forms them
+                    dbuf<RayJob> d_jobs2(redo.size());
+                    dbuf<double> d_path2(3 * (size_t)at2);
+                    TS_HIP(hipMemcpyAsync(d_jobs2.p, redo.data(), redo.size() * sizeof(RayJob), hipMemcpyHostToDevice, st));
+                    ray_trace<<<((unsigned)redo.size() + 63) / 64, 64, 0, st>>>(fwd, tab, d_st.p, d_ev.p, d_jobs2.p,
+                                                                                (int)redo.size(), step_km, d_path2.p,
+                                                                                d_npts.p, d_tcg.p);
+                    TS_HIP(hipGetLastError());
+                    TS_HIP(hipMemcpyAsync(npts_all.data(), d_npts.p, (size_t)nrays * 4, hipMemcpyDeviceToHost, st));
+                    TS_HIP(hipStreamSynchronize(st));
+                    for (const RayJob &j : redo)
+                        if (npts_all[j.ray] == -2) fail(TS_EDEVICE, "ray path exceeded the back-trace bound");
+                    form_rows(d_path2, d_jobs2, redo, npts_all);
                 }
-                if (redo.empty()) break;
-                if (attempt == 1) fail(TS_EDEVICE, "ray path exceeded the back-trace bound");
-                jobs = redo;
-                at = at2;
-                nb = (int)redo.size();
             }
         }
         std::vector<double> tcg(4 * (size_t)nrays);
@@ -455,26 +533,25 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
         for (int r = 0; r < nrays; r++)
             if (R->counts[r] >= 0 && !std::isfinite(tcg[4 * r])) R->counts[r] = -1;  // RayKernels.cs:94
         // rows in arrival order (RayKernels.cs:114)
-        std::vector<int> slot(nrays, -1);
-        for (size_t i = 0; i < counts_sorted.size(); i++) slot[counts_sorted[i]] = (int)i;
         R->row_start.assign(nrays + 1, 0);
-        for (int r = 0; r < nrays; r++)
-            R->row_start[r + 1] = R->row_start[r] + (R->counts[r] > 0 && slot[r] >= 0 ? R->counts[r] : 0);
+        for (int r = 0; r < nrays; r++) R->row_start[r + 1] = R->row_start[r] + std::max(0, R->counts[r]);
         const int64_t nnz = R->row_start[nrays];
-        std::vector<int32_t> all_nodes(nnz);
-        std::vector<float> all_len(nnz);
-        for (int r = 0; r < nrays; r++) {
-            if (R->row_start[r + 1] == R->row_start[r]) continue;
-            const auto &nn = nodes_parts[slot[r]];
-            const auto &ll = len_parts[slot[r]];
-            std::copy(nn.begin(), nn.end(), all_nodes.begin() + R->row_start[r]);
-            std::copy(ll.begin(), ll.end(), all_len.begin() + R->row_start[r]);
-        }
         R->nodes.alloc(std::max<int64_t>(1, nnz));
         R->lengths.alloc(std::max<int64_t>(1, nnz));
-        if (nnz > 0) {
-            TS_HIP(hipMemcpy(R->nodes.p, all_nodes.data(), nnz * 4, hipMemcpyHostToDevice));
-            TS_HIP(hipMemcpy(R->lengths.p, all_len.data(), nnz * 4, hipMemcpyHostToDevice));
+        for (Formed &F : formed) {
+            const int n = (int)F.ray.size();
+            std::vector<int64_t> dst(n);
+            for (int q = 0; q < n; q++) dst[q] = R->row_start[F.ray[q]];
+            std::vector<int32_t> cnt(n);
+            for (int q = 0; q < n; q++) cnt[q] = R->counts[F.ray[q]] > 0 ? F.count[q] : 0;  // rows voided since
+            dbuf<int64_t> d_src(n), d_dst(n);
+            dbuf<int32_t> d_cnt(n);
+            TS_HIP(hipMemcpyAsync(d_src.p, F.at.data(), n * 8, hipMemcpyHostToDevice, st));
+            TS_HIP(hipMemcpyAsync(d_dst.p, dst.data(), n * 8, hipMemcpyHostToDevice, st));
+            TS_HIP(hipMemcpyAsync(d_cnt.p, cnt.data(), n * 4, hipMemcpyHostToDevice, st));
+            ray_move<<<n, 256, 0, st>>>(F.nodes.p, F.len.p, d_src.p, d_dst.p, d_cnt.p, n, R->nodes.p, R->lengths.p);
+            TS_HIP(hipGetLastError());
+            TS_HIP(hipStreamSynchronize(st));
         }
         R->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         *out = R.release();

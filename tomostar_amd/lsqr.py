@@ -14,10 +14,10 @@ from typing import Callable, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import TS_LSQR_FAST_FP32, TS_LSQR_PLAIN_FP64, check, ptr, reason_buffer
+from ._lib import TS_LSQR_FAST_FP32, TS_LSQR_FAST_MIXED, TS_LSQR_PLAIN_FP64, check, ptr, reason_buffer
 from .eikonal import Device
 
-MODES = {"plain_fp64": TS_LSQR_PLAIN_FP64, "fast_fp32": TS_LSQR_FAST_FP32}
+MODES = {"plain_fp64": TS_LSQR_PLAIN_FP64, "fast_fp32": TS_LSQR_FAST_FP32, "fast_mixed": TS_LSQR_FAST_MIXED}
 
 # Lsqr.cs:141-147 and :48-62, in the ABI's stop-code order
 STOP_REASONS = {

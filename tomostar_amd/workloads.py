@@ -152,3 +152,19 @@ def station_seed_sets(wl: Workload, point_source, phases=("P", "S")):
             seeds.append(point_source(wl.grid, models[ph], st[0], st[1], st[2]))
             mof.append(0 if ph == "P" else 1)
     return seeds, np.asarray(mof, np.int32)
+
+
+def scale_arrivals(wl: Workload, events: int = 20000, rays: int = 1_000_000, seed: int = 2):
+    """The LSQR-at-scale arrivals: `events` hypocentres uniform in the box at 3-15 km, and the first `rays`
+    station x event x phase triples of a seeded shuffle. Returns (event positions (E, 3), station,
+    event and phase (0 P, 1 S) of each arrival, in arrival order)."""
+    rng = np.random.default_rng(seed)
+    ev = _uniform_points(rng, wl.grid, events, 3.0, 15.0)
+    ns = len(wl.stations)
+    total = ns * events * 2
+    pick = np.sort(rng.permutation(total)[:min(rays, total)])
+    # triple index = (event * stations + station) * 2 + phase: arrivals grouped by event, as picks are
+    phase = (pick % 2).astype(np.int32)
+    station = ((pick // 2) % ns).astype(np.int32)
+    event = (pick // (2 * ns)).astype(np.int32)
+    return ev, station, event, phase
