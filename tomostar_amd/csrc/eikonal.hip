@@ -212,8 +212,8 @@ struct FastArgs {
     const uint64_t *seedmask;     // [groups][N]: lanes whose field holds this node fixed
     const R *slow;                // [models][N]
     const int32_t *model;         // [groups*64]
-    const R *hlon, *hlat;         // [ny*nz], [nz]
-    R hr;
+    const R *wlon, *wlat;         // spacing weights (first, second order): [ny*nz][2], [nz][2]
+    R wr0, wr1;                   // and of the depth axis
     const uint64_t *active;       // [groups]: lanes still relaxing
     const uint64_t *phase1;       // [groups]: lanes in the second-order phase
     uint64_t *changed;            // [groups]: lanes that moved a node by more than the settle tolerance
@@ -222,16 +222,23 @@ struct FastArgs {
     int nx, ny, nz, N;
 };
 
-template <typename R, int WPB, bool kCount>
-__global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int dir, int off, int cnt)
+// A load at a uniform base plus a 32-bit byte offset (one address register per lane).
+template <typename T>
+__device__ inline T at_off(const T *base, uint32_t byte_off)
 {
-    const int lane = threadIdx.x & 63;
-    const int a = blockIdx.x * WPB + (threadIdx.x >> 6);
-    const int g = blockIdx.y;
-    if (a >= cnt) return;
-    const uint64_t act = A.active[g];
-    if (act == 0) return;
-    const uint32_t pk = A.planes[off + a];
+    return *(const T *)((const char *)base + byte_off);
+}
+
+// The update of plane node a for the 64 fields of group g (one wavefront, one lane per field). With
+// kEarly the dirty flag is read first and a clean node costs one load; otherwise every load of the
+// update - the flag, the seed mask, the slowness, the node and its first and second neighbours on
+// each axis - is issued at once, so a wave waits for memory once. No node of a plane reads another
+// node of the same plane (neighbours lie one or two planes away), so reading early reads the same
+// values.
+template <typename R, bool kCount, bool kOff32, bool kEarly>
+__device__ inline void update_node(const FastArgs<R> &A, int dir, uint32_t pk, int g, uint64_t act, int lane,
+                                   unsigned counter)
+{
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
     const int ip = pk & 2047, jp = (pk >> 11) & 2047, kp = pk >> 22;
     const int i = (dir & 1) ? nx - 1 - ip : ip;
@@ -240,34 +247,53 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int di
     const int node = (k * ny + j) * nx + i;
     const size_t gn = (size_t)g * A.N;
     uint8_t *flag = A.dirty + gn + node;
-    if (*flag == 0) return;  // nothing this node reads has moved: the update would return its value
-    if (lane == 0) *flag = 0;
+    const uint8_t fl = *flag;
+    if (kEarly && fl == 0) return;  // nothing this node reads has moved: the update would return its value
     const uint64_t fixed = A.seedmask[gn + node];
+    const uint64_t ph = A.phase1[g];
+    const bool any_second = (ph & act) != 0;  // wave-uniform
+    R *Tg = A.T + gn * kLanes;                // this group's times (uniform base)
+    auto T = [&](int q) -> R {                // time of node q for this lane's field
+        if constexpr (kOff32) return at_off(Tg, ((uint32_t)q * kLanes + lane) * (uint32_t)sizeof(R));
+        else return Tg[(size_t)q * kLanes + lane];
+    };
+    const int mdl = A.model[g * kLanes + lane];
+    R s;
+    if constexpr (kOff32) s = at_off(A.slow, ((uint32_t)mdl * (uint32_t)A.N + node) * (uint32_t)sizeof(R));
+    else s = A.slow[(size_t)mdl * A.N + node];
+    const R cur = T(node);
+    const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
+    const R wa[3][2] = {{A.wlon[2 * (k * ny + j)], A.wlon[2 * (k * ny + j) + 1]},
+                        {A.wlat[2 * k], A.wlat[2 * k + 1]}, {A.wr0, A.wr1}};
+    R mn[3], pl[3], m2[3], p2[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        mn[ax] = pos[ax] > 0 ? T(node - st[ax]) : (R)INFINITY;
+        pl[ax] = pos[ax] + 1 < n[ax] ? T(node + st[ax]) : (R)INFINITY;
+        m2[ax] = any_second && pos[ax] > 1 ? T(node - 2 * st[ax]) : (R)INFINITY;
+        p2[ax] = any_second && pos[ax] + 2 < n[ax] ? T(node + 2 * st[ax]) : (R)INFINITY;
+    }
+    if (fl == 0) return;
+    if (lane == 0) *flag = 0;
     const bool on = ((act & ~fixed) >> lane) & 1;
-    if (kCount && lane == 0) atomicAdd(&A.evals[blockIdx.x & 1023], (unsigned long long)__popcll(act & ~fixed));
-    R *Tg = A.T + gn * kLanes + lane;
+    if (kCount && lane == 0) atomicAdd(&A.evals[counter & 1023], (unsigned long long)__popcll(act & ~fixed));
     bool dropped = false, moved = false;
     if (on) {
-        const R s = A.slow[(size_t)A.model[g * kLanes + lane] * A.N + node];
-        const bool second = (A.phase1[g] >> lane) & 1;
-        const R cur = Tg[(size_t)node * kLanes];
-        const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
-        const R h[3] = {A.hlon[k * ny + j], A.hlat[k], A.hr};
+        const bool second = (ph >> lane) & 1;
         R first[3], t2[3];
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
-            R mn = pos[ax] > 0 ? Tg[(size_t)(node - st[ax]) * kLanes] : (R)INFINITY;
-            R pl = pos[ax] + 1 < n[ax] ? Tg[(size_t)(node + st[ax]) * kLanes] : (R)INFINITY;
-            int sec;
-            if (mn <= pl) { first[ax] = mn; sec = pos[ax] > 1 ? node - 2 * st[ax] : -1; }
-            else { first[ax] = pl; sec = pos[ax] + 2 < n[ax] ? node + 2 * st[ax] : -1; }
-            t2[ax] = (second && sec >= 0 && isfinite(first[ax])) ? Tg[(size_t)sec * kLanes] : (R)INFINITY;
+            // the smaller neighbour of the axis and, second order, the next node beyond it
+            const bool minus = mn[ax] <= pl[ax];
+            first[ax] = minus ? mn[ax] : pl[ax];
+            const R beyond = minus ? m2[ax] : p2[ax];
+            t2[ax] = (second && isfinite(first[ax])) ? beyond : (R)INFINITY;
         }
         R next;
-        if constexpr (sizeof(R) == 4) next = (s > 0.0f) ? godunov_offset(s, first, t2, h) : (R)INFINITY;
-        else next = (s > 0.0) ? godunov_abs<double>(s, first, t2, h) : (R)INFINITY;
+        if constexpr (sizeof(R) == 4) next = (s > 0.0f) ? godunov_offset(s, first, t2, wa) : (R)INFINITY;
+        else next = (s > 0.0) ? godunov_abs_w<double>(s, first, t2, wa) : (R)INFINITY;
         if (next < cur) {
-            Tg[(size_t)node * kLanes] = next;
+            Tg[(size_t)node * kLanes + lane] = next;
             dropped = true;
             if constexpr (sizeof(R) == 4 && TS_FP32_SETTLE_ULPS > 0)
                 moved = !isfinite(cur) || (cur - next) > fmaxf((float)kSettleTol, TS_FP32_SETTLE_ULPS * 1.1920929e-7f * cur);
@@ -292,6 +318,23 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int di
         uint64_t seen = __hip_atomic_load(&A.changed[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (mv & ~seen) atomicOr((unsigned long long *)&A.changed[g], (unsigned long long)mv);
     }
+}
+
+// One diagonal plane of one sweep ordering for the groups g0 .. g0 + gridDim.y - 1: each wavefront
+// updates plane nodes a, a + W, a + 2W, ... (W the waves of a group's row of blocks), so a launch
+// is a few resident waves per SIMD rather than one short-lived workgroup per node. FP32 times (the
+// first-order phase, where most nodes are clean) read the dirty flag first; FP64 times (the
+// second-order phase, where most are not) issue every load at once.
+template <typename R, int WPB, bool kCount, bool kOff32>
+__global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int dir, int off, int cnt, int g0)
+{
+    const int lane = threadIdx.x & 63;
+    const int g = g0 + blockIdx.y;
+    const uint64_t act = A.active[g];
+    if (act == 0) return;
+    const int stride = gridDim.x * WPB;
+    for (int a = blockIdx.x * WPB + (threadIdx.x >> 6); a < cnt; a += stride)
+        update_node<R, kCount, kOff32, sizeof(R) == 4>(A, dir, A.planes[off + a], g, act, lane, blockIdx.x);
 }
 
 // Per-field bookkeeping after a round (one wave per group): a field whose round moved no node by
@@ -433,6 +476,10 @@ struct ts_tableset {
     double hr = 0;
     dbuf<double> slow64, hlon64, hlat64;  // [models][N]
     dbuf<float> slow32, hlon32, hlat32;
+    dbuf<double> wlon64, wlat64;          // spacing weights of the fast form: [ny*nz][2], [nz][2]
+    dbuf<float> wlon32, wlat32;
+    double wr64[2] = {0, 0};
+    float wr32[2] = {0, 0};
     dbuf<uint32_t> planes;
     std::vector<int> plane_off;           // nx+ny+nz-1 entries
     // results
@@ -443,6 +490,7 @@ struct ts_tableset {
     double evaluations = 0, evaluations32 = 0, evaluations64 = 0, seconds = 0, sweep_seconds = 0;
     double sweep_by_width[2] = {0, 0};  // sweep-graph device seconds on FP32 / FP64 times
     int32_t rounds_by_width[2] = {0, 0};  // rounds run on FP32 / FP64 times
+    int32_t plane_launches_per_round = 0;
     // fast-form working storage, kept across solves
     dbuf<float> T32;
     dbuf<double> T64;
@@ -604,8 +652,11 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
     A.T = T;
     A.dirty = t.dirty.p;
     A.seedmask = t.seedmask.p;
-    if constexpr (sizeof(R) == 4) { A.slow = t.slow32.p; A.hlon = t.hlon32.p; A.hlat = t.hlat32.p; A.hr = (float)t.hr; }
-    else { A.slow = t.slow64.p; A.hlon = t.hlon64.p; A.hlat = t.hlat64.p; A.hr = t.hr; }
+    if constexpr (sizeof(R) == 4) {
+        A.slow = t.slow32.p; A.wlon = t.wlon32.p; A.wlat = t.wlat32.p; A.wr0 = t.wr32[0]; A.wr1 = t.wr32[1];
+    } else {
+        A.slow = t.slow64.p; A.wlon = t.wlon64.p; A.wlat = t.wlat64.p; A.wr0 = t.wr64[0]; A.wr1 = t.wr64[1];
+    }
     A.model = t.model_dev.p;
     A.active = t.active.p;
     A.phase1 = t.phase1.p;
@@ -618,18 +669,47 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
 
 constexpr int kWPB = 4;
 
+// Cap on the blocks of a plane launch over all its groups (TOMOSTAR_PLANE_BLOCKS): waves stride over
+// the plane beyond it. Measured on the 400-field case: fewer, longer-lived waves are slower (the
+// sweep is bound by the memory requests in flight), so by default every plane node gets its own wave.
+int plane_blocks(const ts_tableset &t)
+{
+    (void)t;
+    if (const char *e = getenv("TOMOSTAR_PLANE_BLOCKS")) return std::max(1, atoi(e));
+    return 1 << 30;
+}
+
+// Groups swept together: TOMOSTAR_GROUP_CHUNK when set, else all of them.
+int group_chunk(const ts_tableset &t, size_t width)
+{
+    (void)width;
+    if (const char *e = getenv("TOMOSTAR_GROUP_CHUNK")) return std::max(1, atoi(e));
+    return (t.nfields + kLanes - 1) / kLanes;
+}
+
 template <typename R, bool kCount>
 void launch_round(ts_tableset &t, R *T, hipStream_t st, int hold)
 {
     const int groups = (t.nfields + kLanes - 1) / kLanes;
     FastArgs<R> A = fast_args<R>(t, T);
     const int P = t.grid.nx + t.grid.ny + t.grid.nz - 2;
-    for (int dir = 0; dir < 8; dir++)
-        for (int p = 0; p < P; p++) {
-            int off = t.plane_off[p], cnt = t.plane_off[p + 1] - off;
-            dim3 grid(blocks_for(cnt, kWPB), groups);
-            eik_fast_plane<R, kWPB, kCount><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt);
-        }
+    // groups are independent fields: a chunk of them sweeps all eight orderings before the next chunk
+    // starts, so the planes a sweep keeps re-reading belong to fewer fields and stay in cache
+    const int chunk = std::max(1, std::min(groups, group_chunk(t, sizeof(R))));
+    t.plane_launches_per_round = ((groups + chunk - 1) / chunk) * 8 * P;
+    // 32-bit byte offsets when a group's times and all slowness models fit in 4 GB
+    const bool off32 = (uint64_t)t.N * kLanes * sizeof(R) < (1ull << 32) &&
+                       (uint64_t)t.nmodels * t.N * sizeof(R) < (1ull << 32);
+    for (int g0 = 0; g0 < groups; g0 += chunk) {
+        const int gc = std::min(chunk, groups - g0);
+        for (int dir = 0; dir < 8; dir++)
+            for (int p = 0; p < P; p++) {
+                int off = t.plane_off[p], cnt = t.plane_off[p + 1] - off;
+                dim3 grid(std::min<unsigned>(blocks_for(cnt, kWPB), std::max(1, plane_blocks(t) / gc)), gc);
+                if (off32) eik_fast_plane<R, kWPB, kCount, true><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                else eik_fast_plane<R, kWPB, kCount, false><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+            }
+    }
     eik_fast_round_end<<<groups, 64, 0, st>>>(t.active.p, t.phase1.p, t.changed.p, t.state.p, t.rounds_dev.p,
                                               sweep_round_cap(t.grid), t.status.p, t.reset.p, hold);
     dim3 rg(blocks_for(t.N, 256), groups);
@@ -930,6 +1010,27 @@ int ts_tables_create(ts_context *ctx, const ts_grid *grid, int nmodels, const do
         TS_HIP(hipMemcpy(t->hlat64.p, t->hlat.data(), t->hlat.size() * 8, hipMemcpyHostToDevice));
         TS_HIP(hipMemcpy(t->hlon32.p, hl32.data(), hl32.size() * 4, hipMemcpyHostToDevice));
         TS_HIP(hipMemcpy(t->hlat32.p, ht32.data(), ht32.size() * 4, hipMemcpyHostToDevice));
+        {
+            // the fast form's spacing weights, FP64 from the FP64 metric and FP32 from the FP32 one
+            std::vector<double> wl(2 * t->hlon.size()), wt(2 * t->hlat.size());
+            std::vector<float> wl32(wl.size()), wt32(wt.size());
+            for (size_t q = 0; q < t->hlon.size(); q++) {
+                spacing_weights<double>(t->hlon[q], &wl[2 * q]);
+                spacing_weights<float>(hl32[q], &wl32[2 * q]);
+            }
+            for (size_t q = 0; q < t->hlat.size(); q++) {
+                spacing_weights<double>(t->hlat[q], &wt[2 * q]);
+                spacing_weights<float>(ht32[q], &wt32[2 * q]);
+            }
+            spacing_weights<double>(t->hr, t->wr64);
+            spacing_weights<float>((float)t->hr, t->wr32);
+            t->wlon64.alloc(wl.size()); t->wlat64.alloc(wt.size());
+            t->wlon32.alloc(wl.size()); t->wlat32.alloc(wt.size());
+            TS_HIP(hipMemcpy(t->wlon64.p, wl.data(), wl.size() * 8, hipMemcpyHostToDevice));
+            TS_HIP(hipMemcpy(t->wlat64.p, wt.data(), wt.size() * 8, hipMemcpyHostToDevice));
+            TS_HIP(hipMemcpy(t->wlon32.p, wl32.data(), wl32.size() * 4, hipMemcpyHostToDevice));
+            TS_HIP(hipMemcpy(t->wlat32.p, wt32.data(), wt32.size() * 4, hipMemcpyHostToDevice));
+        }
         build_planes(*t);
         t->rounds.assign(2 * (size_t)nfields, 0);
         *out = t.release();
@@ -1023,7 +1124,7 @@ int ts_tables_sweep_stats(ts_tableset *t, int32_t rounds[2], int32_t *plane_laun
 {
     if (!t) return TS_EINVAL;
     if (rounds) { rounds[0] = t->rounds_by_width[0]; rounds[1] = t->rounds_by_width[1]; }
-    if (plane_launches_per_round) *plane_launches_per_round = 8 * (t->grid.nx + t->grid.ny + t->grid.nz - 2);
+    if (plane_launches_per_round) *plane_launches_per_round = t->plane_launches_per_round;
     if (seconds) { seconds[0] = t->sweep_by_width[0]; seconds[1] = t->sweep_by_width[1]; }
     return TS_OK;
 }
