@@ -231,6 +231,69 @@ def cpu_baseline_iteration(wl, r):
     return sec
 
 
+def uneven_leg(dev, args):
+    """The uneven batch: 64 single-field problems on grids from 41 x 41 x 21 to 161 x 161 x 81 at
+    0.5-2 km, swept together (one launch per sweep step over all problems); the tuner's choice per
+    size class; the mixed form against the FP64 form (Relax to the last bit) per problem; the plain
+    FP64 one-problem-per-solve form timed on a sample of the problems."""
+    from tomostar_amd import eikonal as E
+    from tomostar_amd import workloads as W
+    from tomostar_amd.gridbatch import GridBatch
+
+    t0 = time.perf_counter()
+    probs = W.uneven_batch_case(seed=3, problems=args.uneven_problems)
+    seeds = [E.point_source(p.grid, p.slowness, *p.source) for p in probs]
+    total = sum(p.grid.count for p in probs)
+    log(f"uneven leg: {len(probs)} problems, {total} nodes ({time.perf_counter() - t0:.1f} s)")
+    b = GridBatch(dev, [p.grid for p in probs], [p.slowness for p in probs], seeds)
+    choice = b.tune()
+    rc, why = b.solve("fast_fp32")  # warm-up
+    if rc != 0:
+        raise RuntimeError(f"uneven batch declined: {why}")
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        rc, why = b.solve("fast_fp32")
+        times.append(time.perf_counter() - t0)
+    mixed = [b.read(q) for q in range(len(probs))]
+    rounds = b.rounds.copy()
+    t0 = time.perf_counter()
+    rc, why = b.solve("fast_fp64")
+    t64 = time.perf_counter() - t0
+    errs = []
+    for q, (idx, _) in enumerate(seeds):
+        ref = b.read(q)
+        mask = np.isfinite(ref) & (ref > 0)
+        mask[idx] = False
+        errs.append(float(np.max(np.abs(mixed[q][mask] - ref[mask]) / ref[mask])))
+    b.release()
+    # the plain FP64 form, one problem per solve, on every eighth problem
+    sample = list(range(0, len(probs), 8))
+    ps, pn = 0.0, 0
+    for q in sample:
+        p = probs[q]
+        ts = E.TableSet(dev, p.grid, [p.slowness], [seeds[q]])
+        t0 = time.perf_counter()
+        rc, why = ts.solve("plain_fp64")
+        ps += time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"plain solve declined: {why}")
+        pn += p.grid.count
+        ts.release()
+    value = total / float(np.mean(times)) / 1e9
+    plain = pn / ps / 1e9
+    out = {"workload": f"{len(probs)} problems, grids 41x41x21 - 161x161x81 (axes drawn jointly), 0.5-2 km, "
+                       f"checkerboard / spike / tabular body +-5-10 %, one interior source each",
+           "nodes": total, "value": round(value, 4), "unit": "Gnode/s (solved nodes per second)",
+           "seconds": round(float(np.mean(times)), 3), "fp64_seconds": round(t64, 3),
+           "tuner_nodes_per_thread": {"<2e5 nodes": choice[0], "2e5-8e5": choice[1], ">=8e5": choice[2]},
+           "max_rel_error_vs_fp64": max(errs), "rounds_max": [int(rounds[:, 0].max()), int(rounds[:, 1].max())],
+           "plain_fp64_gpu": {"value": round(plain, 4), "unit": "Gnode/s", "sample": f"problems {sample}"},
+           "speedup_vs_plain_fp64_gpu": round(value / plain, 2)}
+    log(f"uneven leg: {out}")
+    return out
+
+
 def calibrate_copy(mib: int) -> int:
     """A plain device-to-device copy of known size: the reference launch that hardware byte counters
     (FETCH_SIZE, WRITE_SIZE) are scaled by."""
@@ -259,6 +322,7 @@ def main() -> int:
     ap.add_argument("--lsqr-rhs", type=int, default=32)
     ap.add_argument("--lsqr-iterations", type=int, default=200)
     ap.add_argument("--no-iteration", action="store_true", help="skip the one-iteration pipeline leg")
+    ap.add_argument("--uneven-problems", type=int, default=64, help="problems of the uneven-batch leg (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (counter calibration) and exit")
     args = ap.parse_args()
@@ -351,6 +415,7 @@ def main() -> int:
     lsqr = lsqr_leg(dev, wl, ts, args) if args.lsqr_rays > 0 else None
     ts.release()
     it_leg = None if args.no_iteration else iteration_leg(dev, cpu=not args.no_cpu)
+    uneven = uneven_leg(dev, args) if args.uneven_problems > 0 else None
 
     out = {
         "metric": "eikonal Gnode-upd/s + LSQR iters/s vs plain FP64 GPU; p50/p99 solve latency",
@@ -384,6 +449,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "lsqr": lsqr,
         "iteration": it_leg,
+        "uneven": uneven,
     }
     print(json.dumps(out), flush=True)
     dev.close()

@@ -60,6 +60,7 @@ typedef struct ts_context ts_context;
 typedef struct ts_tableset ts_tableset;
 typedef struct ts_matrix ts_matrix;
 typedef struct ts_rays ts_rays;
+typedef struct ts_gridbatch ts_gridbatch;
 
 /* The spherical grid and its metric (SphericalEikonal.cs:15-35 GridMetric): node counts, HLon[Ny*Nz],
  * HLat[Nz], HR, plus the grid bounds the ray tracer needs (GridDefinition, SphericalGrid.cs:13-31). */
@@ -130,6 +131,28 @@ int ts_tables_stats(ts_tableset *t, double *evaluations_fp32, double *evaluation
  * counterpart in the reference: it is the figure a roofline of the sweep kernel is computed from. */
 int ts_tables_sweep_stats(ts_tableset *t, int32_t rounds[2], int32_t *plane_launches_per_round, double seconds[2]);
 void ts_tables_release(ts_tableset *t);
+
+/* ---------------------------------------------------------------- uneven batches
+ * Many single-field problems, each on its own grid (the adaptive and resolution-test runs, where
+ * the reference calls EikonalOpenCl.TrySolve once per grid, EikonalOpenCl.cs:138-343): grids[q],
+ * slowness[q] (nodes of grid q), the seeds of problem q at seed_offsets[q] .. seed_offsets[q+1].
+ * Every sweep step is one launch over all problems. mode: TS_EIK_FAST_FP64 (Relax to the last bit
+ * per problem) or TS_EIK_FAST_FP32 (first order on FP32 times, second order on FP64 times). */
+int ts_gridbatch_create(ts_context *ctx, int nproblems, const ts_grid *grids, const double *const *slowness,
+                        const int64_t *seed_offsets, const int32_t *seed_index, const double *seed_time,
+                        ts_gridbatch **out, char *reason, int reason_len);
+/* Per size class of problem (< 2e5, < 8e5, >= 8e5 nodes), how many nodes a thread updates per step
+ * (1, 2 or 4), chosen by timing one trial round of each class with each choice; choice[3] receives
+ * the picks. No counterpart in the reference: the batch's own tuner. */
+int ts_gridbatch_tune(ts_gridbatch *b, int32_t choice[3]);
+/* rounds_out[2 * nproblems]: rounds of each problem's first- and second-order phases. */
+int ts_gridbatch_solve(ts_gridbatch *b, int mode, const volatile int32_t *cancel, int32_t *rounds_out,
+                       char *reason, int reason_len);
+/* Travel times of problem q (its grid's nodes, +inf unreached), FP64. */
+int ts_gridbatch_read(ts_gridbatch *b, int problem, double *out);
+/* Total nodes, the last solve's wall time and the device time of its sweep rounds (HIP events). */
+int ts_gridbatch_stats(ts_gridbatch *b, int64_t *nodes, double *seconds, double *sweep_seconds);
+void ts_gridbatch_release(ts_gridbatch *b);
 
 /* ---------------------------------------------------------------- rays
  * The device half of RayKernels.Compute (Tomography/RayKernels.cs:30-115): for every arrival r, the

@@ -105,3 +105,15 @@ def test_system_build_marshalling_without_a_device():
     assert L.ts_matrix_shape(None, None, None, None, None) == _lib.TS_EINVAL
     assert L.ts_matrix_read(None, None, None, None, None) == _lib.TS_EINVAL
     assert L.ts_matrix_read_transpose(None, None, None, None) == _lib.TS_EINVAL
+
+
+def test_gridbatch_rejects_without_a_device():
+    from tomostar_amd import _lib
+    from tomostar_amd import gridbatch as Gb
+
+    L = Gb._declare()
+    r = _lib.reason_buffer()
+    assert L.ts_gridbatch_create(None, 1, None, None, None, None, None, None, r, len(r)) == _lib.TS_EINVAL
+    assert L.ts_gridbatch_solve(None, 1, None, None, r, len(r)) == _lib.TS_EINVAL
+    assert L.ts_gridbatch_read(None, 0, None) == _lib.TS_EINVAL
+    assert L.ts_gridbatch_tune(None, None) == _lib.TS_EINVAL

@@ -1,0 +1,632 @@
+// Many eikonal problems, each on its own grid, solved together: the uneven batch of an adaptive or
+// resolution-test run, where the reference calls EikonalOpenCl.TrySolve once per grid
+// (Forward/EikonalOpenCl.cs:138-343) and each call sweeps its own planes.
+//
+// Here every sweep step is one launch over all problems: plane p of ordering dir of every problem
+// that has a plane p. A per-step segment table is the launch's work queue - segment (problem,
+// first node in the problem's plane list, node count, first block) - and a block finds its segment
+// by binary search. Lanes are nodes (one field per problem, so there is nothing to batch across
+// lanes), times are stored per problem in node order. The update, dirty flags, settle rule, round
+// cap and phases are those of Relax (SphericalEikonal.cs:284-396): the FP64 form reproduces Relax to
+// the last bit per problem; the mixed form runs the first-order phase on FP32 times and the
+// second-order phase on FP64 times, as the batched fast form does.
+//
+// A tuner picks, per size class of problem (by node count), how many nodes a thread updates per
+// step (1, 2 or 4), by timing one trial round of the class alone with each choice.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "eikonal_update.hpp"
+#include "geometry.hpp"
+
+namespace ts {
+void grid_metric(const Grid &g, double *hlon, double *hlat, double *hr);
+}
+
+using namespace ts;
+
+struct MgProblem {
+    int nx, ny, nz, nodes_per_thread;
+    int64_t node_base;   // first node in the concatenated arrays
+    int wlon_base, wlat_base;
+    double wr0, wr1;     // depth-axis spacing weights (FP64; the FP32 kernel narrows them)
+    float wr0f, wr1f;
+};
+
+struct MgSeg {
+    int problem, list_off, count, block_start;
+};
+
+struct ts_gridbatch {
+    ts_context *ctx = nullptr;
+    int nprob = 0;
+    std::vector<Grid> grids;
+    std::vector<int64_t> base;            // node base per problem (+ total)
+    std::vector<std::vector<int>> plane_off;
+    std::vector<int64_t> list_base;       // plane-list base per problem
+    std::vector<MgProblem> prob;
+    std::vector<std::vector<int32_t>> seed_node;
+    std::vector<std::vector<double>> seed_time;
+    int maxP = 0;
+    dbuf<uint32_t> planes;
+    dbuf<double> slow64, wlon64, wlat64, T64;
+    dbuf<float> slow32, wlon32, wlat32, T32;
+    dbuf<uint8_t> dirty, seeded;
+    dbuf<MgProblem> prob_dev;
+    dbuf<int32_t> phase, changed, rounds, status, reset, cap;
+    dbuf<MgSeg> segs;
+    std::vector<int> seg_off;             // per plane index: first segment
+    std::vector<int> seg_blocks;          // per plane index: blocks of its launch
+    std::vector<int32_t> rounds_host;
+    std::vector<int> tuned;               // nodes per thread chosen per size class
+    double seconds = 0, sweep_seconds = 0;
+    int last_mode = -1;
+    std::mutex gate;
+};
+
+namespace ts {
+namespace {
+
+constexpr double kMgSettleTol = 1e-7;  // SphericalEikonal.cs:332
+constexpr int kMgBlock = 256;
+
+inline int mg_round_cap(const Grid &g)  // SphericalEikonal.cs:335
+{
+    const int c = (g.nx + g.ny + g.nz) / 4;
+    return c > 32 ? c : 32;
+}
+
+template <typename R>
+struct MgArgs {
+    R *T;
+    uint8_t *dirty;
+    const uint8_t *seeded;
+    const R *slow, *wlon, *wlat;
+    const MgProblem *prob;
+    const uint32_t *planes;
+    const int32_t *phase;
+    int32_t *changed;
+};
+
+template <typename R>
+__device__ inline void mg_update(const MgArgs<R> &A, const MgProblem &P, int q, int second, int dir, uint32_t pk)
+{
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = nx * ny;
+    const int ip = pk & 2047, jp = (pk >> 11) & 2047, kp = pk >> 22;
+    const int i = (dir & 1) ? nx - 1 - ip : ip;
+    const int j = (dir & 2) ? ny - 1 - jp : jp;
+    const int k = (dir & 4) ? nz - 1 - kp : kp;
+    const int node = (k * ny + j) * nx + i;
+    const int64_t at = P.node_base + node;
+    if (A.seeded[at] || !A.dirty[at]) return;
+    A.dirty[at] = 0;
+    const R s = A.slow[at];
+    if (!(s > (R)0)) return;
+    R *Tp = A.T + P.node_base;
+    const R cur = Tp[node];
+    const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
+    R first[3], t2[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        const R mn = pos[ax] > 0 ? Tp[node - st[ax]] : (R)INFINITY;
+        const R pl = pos[ax] + 1 < n[ax] ? Tp[node + st[ax]] : (R)INFINITY;
+        int sec;
+        if (mn <= pl) { first[ax] = mn; sec = pos[ax] > 1 ? node - 2 * st[ax] : -1; }
+        else { first[ax] = pl; sec = pos[ax] + 2 < n[ax] ? node + 2 * st[ax] : -1; }
+        t2[ax] = (second && sec >= 0 && isfinite(first[ax])) ? Tp[sec] : (R)INFINITY;
+    }
+    R next;
+    if constexpr (sizeof(R) == 4) {
+        const float wa[3][2] = {{A.wlon[P.wlon_base + 2 * (k * ny + j)], A.wlon[P.wlon_base + 2 * (k * ny + j) + 1]},
+                                {A.wlat[P.wlat_base + 2 * k], A.wlat[P.wlat_base + 2 * k + 1]}, {P.wr0f, P.wr1f}};
+        next = godunov_offset(s, first, t2, wa);
+    } else {
+        const double wa[3][2] = {{A.wlon[P.wlon_base + 2 * (k * ny + j)], A.wlon[P.wlon_base + 2 * (k * ny + j) + 1]},
+                                 {A.wlat[P.wlat_base + 2 * k], A.wlat[P.wlat_base + 2 * k + 1]}, {P.wr0, P.wr1}};
+        next = godunov_abs_w<double>(s, first, t2, wa);
+    }
+    if (!(next < cur)) return;
+    Tp[node] = next;
+    if (!isfinite(cur) || (double)(cur - next) > kMgSettleTol) atomicOr(&A.changed[q], 1);
+    uint8_t *d = A.dirty + P.node_base;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        if (pos[ax] > 0) d[node - st[ax]] = 1;
+        if (pos[ax] > 1) d[node - 2 * st[ax]] = 1;
+        if (pos[ax] + 1 < n[ax]) d[node + st[ax]] = 1;
+        if (pos[ax] + 2 < n[ax]) d[node + 2 * st[ax]] = 1;
+    }
+}
+
+// One sweep step over every problem: block b serves the segment whose block range holds b.
+template <typename R>
+__global__ __launch_bounds__(kMgBlock) void mg_plane(MgArgs<R> A, const MgSeg *__restrict__ segs, int nsegs, int dir)
+{
+    __shared__ MgSeg sg;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = nsegs - 1;
+        while (lo < hi) {  // the last segment whose first block is <= blockIdx.x
+            const int mid = (lo + hi + 1) >> 1;
+            if (segs[mid].block_start <= (int)blockIdx.x) lo = mid;
+            else hi = mid - 1;
+        }
+        sg = segs[lo];
+    }
+    __syncthreads();
+    const int q = sg.problem;
+    const int ph = A.phase[q];
+    if (ph >= 2) return;  // done, held or unsettled: not swept
+    const MgProblem P = A.prob[q];
+    const int per = P.nodes_per_thread;
+    const int t = (blockIdx.x - sg.block_start) * kMgBlock + threadIdx.x;
+    const int width = ((sg.count + per - 1) / per + kMgBlock - 1) / kMgBlock * kMgBlock;  // threads of the segment
+    for (int r = 0; r < per; r++) {
+        const int a = t + r * width;
+        if (a < sg.count) mg_update<R>(A, P, q, ph, dir, A.planes[sg.list_off + a]);
+    }
+}
+
+// After a round (one thread per problem): a problem whose round moved no node by more than the
+// tolerance ends its phase; one that reaches the round cap is unsettled. phase: 0 first order,
+// 1 second order, 2 done, 3 first order settled and held (hold != 0), 4 unsettled.
+__global__ void mg_round_end(int nprob, int32_t *phase, int32_t *changed, int32_t *rounds, const int32_t *cap,
+                             int32_t *status, int32_t *reset, int hold)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nprob) return;
+    int ph = phase[q];
+    reset[q] = 0;
+    if (ph < 2) {
+        const int r = ++rounds[2 * q + ph];
+        if (!changed[q]) {
+            if (ph == 0) {
+                ph = hold ? 3 : 1;
+                reset[q] = !hold;
+            } else ph = 2;
+        } else if (r >= cap[q]) ph = 4;
+        phase[q] = ph;
+    }
+    changed[q] = 0;
+    if (ph < 2) atomicAdd(&status[0], 1);
+    if (ph == 4) atomicAdd(&status[1], 1);
+}
+
+// every node of the problems flagged in reset[] (or all problems: reset null) dirty again
+__global__ void mg_reset_dirty(uint8_t *dirty, const int64_t *base, int nprob, const int32_t *reset, int64_t total)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    if (!reset || reset[lo]) dirty[q] = 1;
+}
+
+__global__ void mg_promote(int nprob, int32_t *phase, int32_t *changed)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nprob) return;
+    if (phase[q] == 3) phase[q] = 1;
+    changed[q] = 0;
+}
+
+template <typename R>
+__global__ void mg_clear(R *T, uint8_t *dirty, uint8_t *seeded, int64_t total)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    T[q] = (R)INFINITY;
+    dirty[q] = 1;
+    seeded[q] = 0;
+}
+
+template <typename R>
+__global__ void mg_seed(R *T, uint8_t *seeded, const int64_t *at, const double *time, int count)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    T[at[q]] = (R)time[q];
+    seeded[at[q]] = 1;
+}
+
+__global__ void mg_widen(const float *a, double *b, int64_t n)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q < n) b[q] = (double)a[q];
+}
+
+unsigned mg_blocks(int64_t n, int bs) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
+
+int size_class(int64_t nodes) { return nodes < 200000 ? 0 : nodes < 800000 ? 1 : 2; }
+
+// The segment tables of every plane index for the problems in `which` (all when empty).
+void build_segments(ts_gridbatch &b, const std::vector<int> &which)
+{
+    std::vector<MgSeg> all;
+    b.seg_off.assign(b.maxP + 1, 0);
+    b.seg_blocks.assign(b.maxP, 0);
+    std::vector<char> on(b.nprob, which.empty() ? 1 : 0);
+    for (int q : which) on[q] = 1;
+    for (int p = 0; p < b.maxP; p++) {
+        b.seg_off[p] = (int)all.size();
+        int blocks = 0;
+        for (int q = 0; q < b.nprob; q++) {
+            const auto &po = b.plane_off[q];
+            if (!on[q] || p + 1 >= (int)po.size()) continue;
+            const int cnt = po[p + 1] - po[p];
+            if (cnt <= 0) continue;
+            const int per = b.prob[q].nodes_per_thread;
+            const int nb = (int)mg_blocks((cnt + per - 1) / per, kMgBlock);
+            all.push_back(MgSeg{q, (int)(b.list_base[q] + po[p]), cnt, blocks});
+            blocks += nb;
+        }
+        b.seg_blocks[p] = blocks;
+    }
+    b.seg_off[b.maxP] = (int)all.size();
+    b.segs.alloc(std::max<size_t>(1, all.size()));
+    if (!all.empty()) TS_HIP(hipMemcpy(b.segs.p, all.data(), all.size() * sizeof(MgSeg), hipMemcpyHostToDevice));
+}
+
+template <typename R>
+MgArgs<R> mg_args(ts_gridbatch &b)
+{
+    MgArgs<R> A;
+    if constexpr (sizeof(R) == 4) { A.T = b.T32.p; A.slow = b.slow32.p; A.wlon = b.wlon32.p; A.wlat = b.wlat32.p; }
+    else { A.T = b.T64.p; A.slow = b.slow64.p; A.wlon = b.wlon64.p; A.wlat = b.wlat64.p; }
+    A.dirty = b.dirty.p;
+    A.seeded = b.seeded.p;
+    A.prob = b.prob_dev.p;
+    A.planes = b.planes.p;
+    A.phase = b.phase.p;
+    A.changed = b.changed.p;
+    return A;
+}
+
+template <typename R>
+void mg_round(ts_gridbatch &b, hipStream_t st, int hold)
+{
+    const MgArgs<R> A = mg_args<R>(b);
+    for (int dir = 0; dir < 8; dir++)
+        for (int p = 0; p < b.maxP; p++) {
+            const int ns = b.seg_off[p + 1] - b.seg_off[p];
+            if (ns == 0 || b.seg_blocks[p] == 0) continue;
+            mg_plane<R><<<b.seg_blocks[p], kMgBlock, 0, st>>>(A, b.segs.p + b.seg_off[p], ns, dir);
+        }
+    mg_round_end<<<mg_blocks(b.nprob, 64), 64, 0, st>>>(b.nprob, b.phase.p, b.changed.p, b.rounds.p, b.cap.p,
+                                                       b.status.p, b.reset.p, hold);
+}
+
+void mg_start(ts_gridbatch &b, bool fp32, hipStream_t st)
+{
+    const int64_t total = b.base[b.nprob];
+    if (fp32) mg_clear<float><<<mg_blocks(total, 256), 256, 0, st>>>(b.T32.p, b.dirty.p, b.seeded.p, total);
+    else mg_clear<double><<<mg_blocks(total, 256), 256, 0, st>>>(b.T64.p, b.dirty.p, b.seeded.p, total);
+    std::vector<int64_t> at;
+    std::vector<double> tm;
+    for (int q = 0; q < b.nprob; q++)
+        for (size_t s = 0; s < b.seed_node[q].size(); s++) {
+            at.push_back(b.base[q] + b.seed_node[q][s]);
+            tm.push_back(b.seed_time[q][s]);
+        }
+    dbuf<int64_t> d_at(std::max<size_t>(1, at.size()));
+    dbuf<double> d_tm(std::max<size_t>(1, tm.size()));
+    if (!at.empty()) {
+        TS_HIP(hipMemcpyAsync(d_at.p, at.data(), at.size() * 8, hipMemcpyHostToDevice, st));
+        TS_HIP(hipMemcpyAsync(d_tm.p, tm.data(), tm.size() * 8, hipMemcpyHostToDevice, st));
+        if (fp32) mg_seed<float><<<mg_blocks(at.size(), 256), 256, 0, st>>>(b.T32.p, b.seeded.p, d_at.p, d_tm.p, (int)at.size());
+        else mg_seed<double><<<mg_blocks(at.size(), 256), 256, 0, st>>>(b.T64.p, b.seeded.p, d_at.p, d_tm.p, (int)at.size());
+    }
+    TS_HIP(hipMemsetAsync(b.phase.p, 0, b.nprob * 4, st));
+    TS_HIP(hipMemsetAsync(b.changed.p, 0, b.nprob * 4, st));
+    TS_HIP(hipMemsetAsync(b.rounds.p, 0, 2 * b.nprob * 4, st));
+    TS_HIP(hipGetLastError());
+    TS_HIP(hipStreamSynchronize(st));
+}
+
+// Rounds until no problem relaxes; declines when a problem reaches its round cap.
+template <typename R>
+void mg_rounds(ts_gridbatch &b, const volatile int32_t *cancel, int hold, int max_rounds, int32_t *h_status,
+               const int64_t *d_base)
+{
+    hipStream_t st = b.ctx->stream;
+    hipEvent_t e0, e1;
+    TS_HIP(hipEventCreate(&e0));
+    TS_HIP(hipEventCreate(&e1));
+    struct FreeEv { hipEvent_t a, c; ~FreeEv() { (void)hipEventDestroy(a); (void)hipEventDestroy(c); } } fe{e0, e1};
+    const int64_t total = b.base[b.nprob];
+    for (int round = 0; round < max_rounds; round++) {
+        if (cancel && *cancel) fail(TS_CANCELLED, "cancelled");
+        TS_HIP(hipMemsetAsync(b.status.p, 0, 8, st));
+        TS_HIP(hipEventRecord(e0, st));
+        mg_round<R>(b, st, hold);
+        TS_HIP(hipEventRecord(e1, st));
+        if (!hold) mg_reset_dirty<<<mg_blocks(total, 256), 256, 0, st>>>(b.dirty.p, d_base, b.nprob, b.reset.p, total);
+        TS_HIP(hipMemcpyAsync(h_status, b.status.p, 8, hipMemcpyDeviceToHost, st));
+        TS_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        TS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        b.sweep_seconds += ms * 1e-3;
+        if (h_status[1] > 0) fail(TS_DECLINED, "%d problem(s) not settled within their round cap", h_status[1]);
+        if (h_status[0] == 0) return;
+    }
+    fail(TS_DECLINED, "sweeping not settled");
+}
+
+void mg_solve(ts_gridbatch &b, int mode, const volatile int32_t *cancel)
+{
+    hipStream_t st = b.ctx->stream;
+    const bool mixed = mode == TS_EIK_FAST_FP32;
+    const int64_t total = b.base[b.nprob];
+    dbuf<int64_t> d_base(b.nprob);
+    TS_HIP(hipMemcpy(d_base.p, b.base.data(), b.nprob * 8, hipMemcpyHostToDevice));
+    int32_t *h_status = nullptr;
+    TS_HIP(hipHostMalloc((void **)&h_status, 8, hipHostMallocDefault));
+    struct FreeH { int32_t *p; ~FreeH() { if (p) (void)hipHostFree(p); } } fh{h_status};
+    int max_rounds = 0;
+    for (const Grid &g : b.grids) max_rounds = std::max(max_rounds, 2 * mg_round_cap(g) + 2);
+    b.sweep_seconds = 0;
+    mg_start(b, mixed, st);
+    if (mixed) {
+        mg_rounds<float>(b, cancel, 1, max_rounds, h_status, d_base.p);
+        // second-order phase on FP64 times from the first-order FP32 times; the seeds as given
+        std::vector<int64_t> at;
+        std::vector<double> tm;
+        for (int q = 0; q < b.nprob; q++)
+            for (size_t s = 0; s < b.seed_node[q].size(); s++) {
+                at.push_back(b.base[q] + b.seed_node[q][s]);
+                tm.push_back(b.seed_time[q][s]);
+            }
+        mg_widen<<<mg_blocks(total, 256), 256, 0, st>>>(b.T32.p, b.T64.p, total);
+        dbuf<int64_t> d_at(std::max<size_t>(1, at.size()));
+        dbuf<double> d_tm(std::max<size_t>(1, tm.size()));
+        if (!at.empty()) {
+            TS_HIP(hipMemcpyAsync(d_at.p, at.data(), at.size() * 8, hipMemcpyHostToDevice, st));
+            TS_HIP(hipMemcpyAsync(d_tm.p, tm.data(), tm.size() * 8, hipMemcpyHostToDevice, st));
+            mg_seed<double><<<mg_blocks(at.size(), 256), 256, 0, st>>>(b.T64.p, b.seeded.p, d_at.p, d_tm.p, (int)at.size());
+        }
+        mg_promote<<<mg_blocks(b.nprob, 64), 64, 0, st>>>(b.nprob, b.phase.p, b.changed.p);
+        mg_reset_dirty<<<mg_blocks(total, 256), 256, 0, st>>>(b.dirty.p, d_base.p, b.nprob, nullptr, total);
+        TS_HIP(hipGetLastError());
+        TS_HIP(hipStreamSynchronize(st));
+    }
+    mg_rounds<double>(b, cancel, 0, max_rounds, h_status, d_base.p);
+    TS_HIP(hipMemcpy(b.rounds_host.data(), b.rounds.p, 2 * b.nprob * 4, hipMemcpyDeviceToHost));
+}
+
+// Per size class, the nodes each thread updates per step: one trial first-order round of the class
+// alone with each choice, timed with HIP events; the fastest is kept.
+void mg_tune(ts_gridbatch &b)
+{
+    hipStream_t st = b.ctx->stream;
+    b.tuned.assign(3, 1);
+    const int choices[3] = {1, 2, 4};
+    for (int c = 0; c < 3; c++) {
+        std::vector<int> which;
+        for (int q = 0; q < b.nprob; q++)
+            if (size_class(b.base[q + 1] - b.base[q]) == c) which.push_back(q);
+        if (which.empty()) continue;
+        double best = 1e300;
+        for (int per : choices) {
+            for (int q : which) b.prob[q].nodes_per_thread = per;
+            TS_HIP(hipMemcpy(b.prob_dev.p, b.prob.data(), b.nprob * sizeof(MgProblem), hipMemcpyHostToDevice));
+            build_segments(b, which);
+            mg_start(b, true, st);
+            hipEvent_t e0, e1;
+            TS_HIP(hipEventCreate(&e0));
+            TS_HIP(hipEventCreate(&e1));
+            TS_HIP(hipEventRecord(e0, st));
+            mg_round<float>(b, st, 1);
+            TS_HIP(hipEventRecord(e1, st));
+            TS_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            TS_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (ms < best) { best = ms; b.tuned[c] = per; }
+        }
+        for (int q : which) b.prob[q].nodes_per_thread = b.tuned[c];
+    }
+    TS_HIP(hipMemcpy(b.prob_dev.p, b.prob.data(), b.nprob * sizeof(MgProblem), hipMemcpyHostToDevice));
+    build_segments(b, {});
+}
+
+}  // namespace
+}  // namespace ts
+
+// ================================================================================== C ABI
+
+extern "C" {
+
+int ts_gridbatch_create(ts_context *ctx, int nproblems, const ts_grid *grids, const double *const *slowness,
+                        const int64_t *seed_offsets, const int32_t *seed_index, const double *seed_time,
+                        ts_gridbatch **out, char *reason, int reason_len)
+{
+    if (out) *out = nullptr;
+    if (!ctx || nproblems < 1 || !grids || !slowness || !seed_offsets || !out) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    return guarded(reason, reason_len, [&]() -> int {
+        TS_HIP(hipSetDevice(ctx->device));
+        std::unique_ptr<ts_gridbatch> B(new ts_gridbatch);
+        ts_gridbatch &b = *B;
+        b.ctx = ctx;
+        b.nprob = nproblems;
+        b.base.assign(nproblems + 1, 0);
+        b.list_base.assign(nproblems, 0);
+        b.seed_node.resize(nproblems);
+        b.seed_time.resize(nproblems);
+        b.plane_off.resize(nproblems);
+        std::vector<uint32_t> list;
+        std::vector<double> wl, wt;
+        std::vector<float> wl32, wt32;
+        for (int q = 0; q < nproblems; q++) {
+            if (!grid_valid(grids[q])) fail(TS_EINVAL, "grid %d is invalid", q);
+            if (!slowness[q]) fail(TS_EINVAL, "slowness of problem %d is null", q);
+            const Grid g = make_grid(grids[q]);
+            b.grids.push_back(g);
+            const int N = g.count();
+            b.base[q + 1] = b.base[q] + N;
+            // seeds: in-grid, the smallest time per node
+            std::vector<std::pair<int32_t, double>> sd;
+            for (int64_t s = seed_offsets[q]; s < seed_offsets[q + 1]; s++)
+                if (seed_index[s] >= 0 && seed_index[s] < N) sd.emplace_back(seed_index[s], seed_time[s]);
+            std::sort(sd.begin(), sd.end());
+            for (size_t s = 0; s < sd.size(); s++)
+                if (s == 0 || sd[s].first != sd[s - 1].first) {
+                    b.seed_node[q].push_back(sd[s].first);
+                    b.seed_time[q].push_back(sd[s].second);
+                }
+            if (b.seed_node[q].empty()) fail(TS_DECLINED, "problem %d has no seed node on its grid", q);
+            // plane lists (as the batched fast form's), concatenated
+            const int P = g.nx + g.ny + g.nz - 2;
+            b.maxP = std::max(b.maxP, P);
+            std::vector<int> &po = b.plane_off[q];
+            po.assign(P + 1, 0);
+            b.list_base[q] = (int64_t)list.size();
+            for (int p = 0; p < P; p++) {
+                po[p] = (int)(list.size() - b.list_base[q]);
+                const int i0 = std::max(0, p - (g.ny - 1) - (g.nz - 1)), i1 = std::min(g.nx - 1, p);
+                const int k0 = std::max(0, p - (g.nx - 1) - (g.ny - 1)), k1 = std::min(g.nz - 1, p);
+                for (int kp = k0; kp <= k1; kp++) {
+                    const int lo = std::max(i0, p - kp - (g.ny - 1)), hi = std::min(i1, p - kp);
+                    for (int ip = lo; ip <= hi; ip++)
+                        list.push_back((uint32_t)ip | ((uint32_t)(p - ip - kp) << 11) | ((uint32_t)kp << 22));
+                }
+            }
+            po[P] = (int)(list.size() - b.list_base[q]);
+            // spacing weights of the problem's metric
+            std::vector<double> hl((size_t)g.ny * g.nz), ht(g.nz);
+            double hr = 0;
+            grid_metric(g, hl.data(), ht.data(), &hr);
+            MgProblem P_{};
+            P_.nx = g.nx; P_.ny = g.ny; P_.nz = g.nz; P_.nodes_per_thread = 1;
+            P_.node_base = b.base[q];
+            P_.wlon_base = (int)wl.size();
+            P_.wlat_base = (int)wt.size();
+            for (double h : hl) {
+                double w[2]; float wf[2];
+                spacing_weights<double>(h, w);
+                spacing_weights<float>((float)h, wf);
+                wl.push_back(w[0]); wl.push_back(w[1]);
+                wl32.push_back(wf[0]); wl32.push_back(wf[1]);
+            }
+            for (double h : ht) {
+                double w[2]; float wf[2];
+                spacing_weights<double>(h, w);
+                spacing_weights<float>((float)h, wf);
+                wt.push_back(w[0]); wt.push_back(w[1]);
+                wt32.push_back(wf[0]); wt32.push_back(wf[1]);
+            }
+            double w[2]; float wf[2];
+            spacing_weights<double>(hr, w);
+            spacing_weights<float>((float)hr, wf);
+            P_.wr0 = w[0]; P_.wr1 = w[1]; P_.wr0f = wf[0]; P_.wr1f = wf[1];
+            b.prob.push_back(P_);
+        }
+        const int64_t total = b.base[nproblems];
+        b.planes.alloc(list.size());
+        TS_HIP(hipMemcpy(b.planes.p, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+        b.slow64.alloc(total);
+        b.slow32.alloc(total);
+        std::vector<float> s32;
+        for (int q = 0; q < nproblems; q++) {
+            const int64_t N = b.base[q + 1] - b.base[q];
+            TS_HIP(hipMemcpy(b.slow64.p + b.base[q], slowness[q], N * 8, hipMemcpyHostToDevice));
+            s32.assign(slowness[q], slowness[q] + N);
+            TS_HIP(hipMemcpy(b.slow32.p + b.base[q], s32.data(), N * 4, hipMemcpyHostToDevice));
+        }
+        b.wlon64.alloc(wl.size()); b.wlat64.alloc(wt.size());
+        b.wlon32.alloc(wl.size()); b.wlat32.alloc(wt.size());
+        TS_HIP(hipMemcpy(b.wlon64.p, wl.data(), wl.size() * 8, hipMemcpyHostToDevice));
+        TS_HIP(hipMemcpy(b.wlat64.p, wt.data(), wt.size() * 8, hipMemcpyHostToDevice));
+        TS_HIP(hipMemcpy(b.wlon32.p, wl32.data(), wl32.size() * 4, hipMemcpyHostToDevice));
+        TS_HIP(hipMemcpy(b.wlat32.p, wt32.data(), wt32.size() * 4, hipMemcpyHostToDevice));
+        b.T64.alloc(total);
+        b.T32.alloc(total);
+        b.dirty.alloc(total);
+        b.seeded.alloc(total);
+        b.prob_dev.alloc(nproblems);
+        b.phase.alloc(nproblems); b.changed.alloc(nproblems); b.rounds.alloc(2 * nproblems);
+        b.reset.alloc(nproblems); b.cap.alloc(nproblems); b.status.alloc(2);
+        std::vector<int32_t> caps(nproblems);
+        for (int q = 0; q < nproblems; q++) caps[q] = mg_round_cap(b.grids[q]);
+        TS_HIP(hipMemcpy(b.cap.p, caps.data(), nproblems * 4, hipMemcpyHostToDevice));
+        TS_HIP(hipMemcpy(b.prob_dev.p, b.prob.data(), nproblems * sizeof(MgProblem), hipMemcpyHostToDevice));
+        build_segments(b, {});
+        b.rounds_host.assign(2 * (size_t)nproblems, 0);
+        *out = B.release();
+        return TS_OK;
+    });
+}
+
+int ts_gridbatch_tune(ts_gridbatch *b, int32_t choice[3])
+{
+    if (!b) return TS_EINVAL;
+    std::lock_guard<std::mutex> lk(b->gate);
+    return guarded(nullptr, 0, [&]() -> int {
+        TS_HIP(hipSetDevice(b->ctx->device));
+        mg_tune(*b);
+        if (choice) for (int c = 0; c < 3; c++) choice[c] = b->tuned[c];
+        return TS_OK;
+    });
+}
+
+int ts_gridbatch_solve(ts_gridbatch *b, int mode, const volatile int32_t *cancel, int32_t *rounds_out, char *reason,
+                       int reason_len)
+{
+    if (!b || (mode != TS_EIK_FAST_FP32 && mode != TS_EIK_FAST_FP64)) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(b->gate);
+    return guarded(reason, reason_len, [&]() -> int {
+        TS_HIP(hipSetDevice(b->ctx->device));
+        auto t0 = std::chrono::steady_clock::now();
+        b->last_mode = -1;
+        mg_solve(*b, mode, cancel);
+        b->last_mode = mode;
+        b->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rounds_out) std::copy(b->rounds_host.begin(), b->rounds_host.end(), rounds_out);
+        return TS_OK;
+    });
+}
+
+int ts_gridbatch_read(ts_gridbatch *b, int problem, double *out)
+{
+    if (!b || problem < 0 || problem >= b->nprob || !out) return TS_EINVAL;
+    std::lock_guard<std::mutex> lk(b->gate);
+    return guarded(nullptr, 0, [&]() -> int {
+        if (b->last_mode < 0) fail(TS_EINVAL, "the batch has not been solved");
+        TS_HIP(hipSetDevice(b->ctx->device));
+        const int64_t N = b->base[problem + 1] - b->base[problem];
+        TS_HIP(hipMemcpy(out, b->T64.p + b->base[problem], N * 8, hipMemcpyDeviceToHost));
+        return TS_OK;
+    });
+}
+
+int ts_gridbatch_stats(ts_gridbatch *b, int64_t *nodes, double *seconds, double *sweep_seconds)
+{
+    if (!b) return TS_EINVAL;
+    if (nodes) *nodes = b->base[b->nprob];
+    if (seconds) *seconds = b->seconds;
+    if (sweep_seconds) *sweep_seconds = b->sweep_seconds;
+    return TS_OK;
+}
+
+void ts_gridbatch_release(ts_gridbatch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    delete b;
+}
+
+}  // extern "C"

@@ -168,3 +168,49 @@ def scale_arrivals(wl: Workload, events: int = 20000, rays: int = 1_000_000, see
     station = ((pick // 2) % ns).astype(np.int32)
     event = (pick // (2 * ns)).astype(np.int32)
     return ev, station, event, phase
+
+
+@dataclass
+class UnevenProblem:
+    grid: GridDefinition
+    slowness: np.ndarray  # P slowness, nodes of grid
+    source: tuple         # lon, lat, depth
+    pattern: str
+    spacing_km: float
+
+
+def uneven_batch_case(seed: int = 3, problems: int = 64, min_h: int = 41, max_h: int = 161, min_v: int = 21,
+                      max_v: int = 81):
+    """The uneven batch: per problem t ~ U[0, 1] sets Nx = Ny = round(min_h + (max_h - min_h) t) and
+    Nz = round(min_v + (max_v - min_v) t) (the axes are drawn jointly), spacing U[0.5, 2] km; Vp =
+    5.0 + 0.04 z times (1 + one of a checkerboard, a Gaussian spike or a tabular body of +-U[5, 10] %,
+    ResolutionTests.cs:22-94 patterns); the source is uniform in the interior, at least 3 cells from
+    every face. One P field per problem."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(problems):
+        t = rng.random()
+        nh = int(round(min_h + (max_h - min_h) * t))
+        nv = int(round(min_v + (max_v - min_v) * t))
+        h = rng.uniform(0.5, 2.0)
+        g0 = box_grid(13.0 + 0.5 * rng.random(), 42.5 + 0.5 * rng.random(), (nh - 1) * h, (nv - 1) * h, h)
+        g = GridDefinition(g0.min_lon, g0.max_lon, g0.min_lat, g0.max_lat, g0.min_depth, g0.max_depth, nh, nh, nv)
+        z = g.depth()[:, None, None]
+        amp = rng.uniform(0.05, 0.10) * rng.choice([-1.0, 1.0])
+        kind = ["checkerboard", "spike", "tabular"][q % 3]
+        if kind == "checkerboard":
+            cell = rng.uniform(5.0, 15.0) * max(1.0, h)
+            pert = checkerboard(g, cell, cell, amp)
+        elif kind == "spike":
+            pert = spike(g, rng.uniform(g.min_lon, g.max_lon), rng.uniform(g.min_lat, g.max_lat),
+                         rng.uniform(0.2, 0.8) * g.max_depth, 0.1 * (nh - 1) * h, amp)
+        else:
+            pert = ellipsoid_body(g, rng.uniform(g.min_lon, g.max_lon), rng.uniform(g.min_lat, g.max_lat),
+                                  rng.uniform(0.3, 0.7) * g.max_depth, 0.25 * (nh - 1) * h, 0.15 * (nh - 1) * h,
+                                  0.05 * (nv - 1) * h, amp)
+        vp = np.ascontiguousarray(np.broadcast_to((5.0 + 0.04 * z) * (1 + pert), g.shape), dtype=np.float64)
+        lon = g.min_lon + (g.max_lon - g.min_lon) * rng.uniform(3 / (nh - 1), 1 - 3 / (nh - 1))
+        lat = g.min_lat + (g.max_lat - g.min_lat) * rng.uniform(3 / (nh - 1), 1 - 3 / (nh - 1))
+        dep = g.min_depth + (g.max_depth - g.min_depth) * rng.uniform(3 / (nv - 1), 1 - 3 / (nv - 1))
+        out.append(UnevenProblem(g, (1.0 / vp).ravel(), (lon, lat, dep), kind, h))
+    return out
