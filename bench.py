@@ -294,6 +294,145 @@ def uneven_leg(dev, args):
     return out
 
 
+def serving_leg(dev, wl, ts, args):
+    """Serving: seeded Poisson arrivals, 70 % table requests (grids 61^3 - 201 x 201 x 101, 1-50 sources,
+    P and S) and 30 % LSQR requests (G of 1e5 - 1e6 rays through the batched-eikonal model, 1-8 RHS,
+    damping log-uniform 0.1-100, 50 iterations); models and matrices resident; admission by a device
+    memory budget. The plain FP64 form serves the same requests one at a time: its service times are
+    measured request by request and its latencies replayed through a single FIFO server at the same
+    arrivals. The arrival rate puts the plain form at 80 % utilisation."""
+    from tomostar_amd import lsqr as Q
+    from tomostar_amd import rays as Ry
+    from tomostar_amd import system as Sy
+    from tomostar_amd import workloads as W
+    from tomostar_amd.serving import Server
+
+    rng = np.random.default_rng(4)
+    budget = args.serving_budget_gb
+    # resident models: four grid sizes of the batched-eikonal kind
+    sizes = [(60.0, 30.0, 1.0), (100.0, 50.0, 1.0), (100.0, 50.0, 0.667), (100.0, 50.0, 0.5)]
+    grids, models_p, models_s = [], [], []
+    for w, d, h in sizes[:-1]:
+        m = W.batched_eikonal_case(seed=5, stations=1, width_km=w, depth_km=d, spacing_km=h)
+        grids.append(m.grid); models_p.append(m.slowness_p); models_s.append(m.slowness_s)
+    grids.append(wl.grid); models_p.append(wl.slowness_p); models_s.append(wl.slowness_s)
+    # resident matrices: G of 1e5, 3e5 and 1e6 rays through the model's tables
+    mats = []
+    for nr in (100_000, 300_000, 1_000_000):
+        ev, st_idx, ev_idx, ph = W.scale_arrivals(wl, events=20000, rays=nr, seed=6 + nr % 7)
+        b = Ry.RayBatch(ts, wl.grid, np.repeat(wl.stations, 2, axis=0), 2 * st_idx + ph, ev[ev_idx], host_rows=False)
+        used = np.flatnonzero(b.counts >= 0).astype(np.int32)
+        sysd = Sy.build_system(dev, wl.grid, Sy.SystemSettings(vpvs=False, smoothing_method="gradient",
+                                                               damping_hypocentre=0.0, damping_station=0.0),
+                               b, used, ph[used], ev_idx[used], st_idx[used], 0.05 * rng.standard_normal(used.size),
+                               np.full(used.size, 0.05), np.zeros(len(ev), np.uint8), np.zeros(len(ev), np.uint8),
+                               len(wl.stations), wl.slowness_p, wl.slowness_s, wl.slowness_p, wl.slowness_s,
+                               wl.slowness_s / wl.slowness_p, damping=0.0, smoothing=1.0, keep_fp64=True)
+        b.release()
+        mats.append((sysd.matrix, sysd.rhs))
+    log(f"serving leg: models {[g.count for g in grids]}, matrices {[a.nnz for a, _ in mats]} non-zeros")
+    # the request stream
+    nreq = args.serving_requests
+    reqs = []
+    for q in range(nreq):
+        if rng.random() < 0.7:
+            gi = int(rng.integers(0, len(grids)))
+            g = grids[gi]
+            ns = int(rng.integers(1, 51))
+            src = np.stack([rng.uniform(g.min_lon, g.max_lon, ns), rng.uniform(g.min_lat, g.max_lat, ns),
+                            rng.uniform(0.0, 0.3 * g.max_depth, ns)], 1)
+            reqs.append(("tables", gi, src))
+        else:
+            mi = int(rng.integers(0, len(mats)))
+            k = int(rng.integers(1, 9))
+            a, rhs = mats[mi]
+            B = rhs[None, :] + 0.05 * rng.standard_normal((k, rhs.size))
+            reqs.append(("lsqr", mi, B, 10.0 ** rng.uniform(-1, 2, k)))
+
+    def serve(server, ids, modes, arrivals=None):
+        """Submits every request (at its arrival time when given), waits for all; per request
+        (submitted, started, finished)."""
+        t0 = time.perf_counter()
+        rids = []
+        for q, r in enumerate(reqs):
+            if arrivals is not None:
+                delay = arrivals[q] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+            if r[0] == "tables":
+                rids.append(server.submit_tables(ids[r[1]][0], ids[r[1]][1], r[2], modes[0]))
+            else:
+                rids.append(server.submit_lsqr(mats[r[1]][0], r[2], r[3], 50, mode=modes[1]))
+        times = []
+        for rid in rids:
+            st, why, t = server.wait(rid, 1200.0)
+            if st != 0:
+                raise RuntimeError(f"request declined: {why}")
+            times.append(t)
+            server.release_request(rid)
+        return np.array(times)
+
+    def open_server(workers):
+        srv = Server(dev, workers=workers, memory_budget_gb=budget)
+        ids = [(srv.add_model(g, p), srv.add_model(g, s_)) for g, p, s_ in zip(grids, models_p, models_s)]
+        return srv, ids
+
+    # plain FP64, one request at a time: service times, then the FIFO replay at the arrivals below
+    srv, ids = open_server(1)
+    t_plain = serve(srv, ids, ("plain_fp64", "plain_fp64"))
+    srv.close()
+    service = t_plain[:, 2] - t_plain[:, 1]
+    lam = 0.8 / float(np.mean(service))
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
+    finish, plain_lat = 0.0, []
+    for q in range(nreq):
+        finish = max(arrivals[q], finish) + service[q]
+        plain_lat.append(finish - arrivals[q])
+    # the fast forms, four workers, the same arrivals in real time
+    srv, ids = open_server(args.serving_workers)
+    serve(srv, ids, ("fast_fp32", "fast_mixed"))  # warm
+    t_fast = serve(srv, ids, ("fast_fp32", "fast_mixed"), arrivals=arrivals)
+    stats = srv.stats()
+    # identity: the first table and LSQR requests again, alone
+    ident = True
+    for r_i, r in enumerate(reqs):
+        if r[0] != "tables":
+            continue
+        rid = srv.submit_tables(ids[r[1]][0], ids[r[1]][1], r[2], "fast_fp32")
+        srv.wait(rid)
+        alone = srv.tables(rid, 0, min(4, 2 * len(r[2])), grids[r[1]].count)
+        srv.release_request(rid)
+        rid = srv.submit_tables(ids[r[1]][0], ids[r[1]][1], r[2], "fast_fp32")
+        rid2 = srv.submit_lsqr(mats[0][0], mats[0][1][None, :], [1.0], 50, mode="fast_mixed")
+        srv.wait(rid)
+        srv.wait(rid2)
+        mixed = srv.tables(rid, 0, min(4, 2 * len(r[2])), grids[r[1]].count)
+        srv.release_request(rid)
+        srv.release_request(rid2)
+        ident = ident and bool(np.array_equal(alone, mixed))
+        break
+    srv.close()
+    for a, _ in mats:
+        a.release()
+    fast_lat = t_fast[:, 2] - t_fast[:, 0]
+    makespan = float(t_fast[:, 2].max() - t_fast[:, 0].min())
+    out = {"workload": f"{nreq} requests, Poisson arrivals at {lam:.3f}/s (80 % utilisation of the plain FP64 "
+                       f"form), 70 % tables (grids {[g.count for g in grids]} nodes, 1-50 sources, P and S), 30 % "
+                       f"LSQR (G of 1e5 / 3e5 / 1e6 rays, 1-8 RHS, damping 0.1-100, 50 iterations)",
+           "memory_budget_gb": budget, "workers": args.serving_workers,
+           "latency_s": {"p50": float(np.percentile(fast_lat, 50)), "p99": float(np.percentile(fast_lat, 99))},
+           "throughput_req_per_s": round(nreq / makespan, 4),
+           "plain_fp64_gpu": {"latency_s": {"p50": float(np.percentile(plain_lat, 50)),
+                                            "p99": float(np.percentile(plain_lat, 99))},
+                              "mean_service_s": float(np.mean(service)),
+                              "how": "service times measured one request at a time, latencies replayed through "
+                                     "one FIFO server at the same arrivals"},
+           "peak_admitted_gb": round(stats["peak_bytes"] / 1e9, 2), "peak_inflight": stats["peak_inflight"],
+           "same_result_alone_and_in_mix": ident}
+    log(f"serving leg: {out}")
+    return out
+
+
 def calibrate_copy(mib: int) -> int:
     """A plain device-to-device copy of known size: the reference launch that hardware byte counters
     (FETCH_SIZE, WRITE_SIZE) are scaled by."""
@@ -323,6 +462,9 @@ def main() -> int:
     ap.add_argument("--lsqr-iterations", type=int, default=200)
     ap.add_argument("--no-iteration", action="store_true", help="skip the one-iteration pipeline leg")
     ap.add_argument("--uneven-problems", type=int, default=64, help="problems of the uneven-batch leg (0 skips it)")
+    ap.add_argument("--serving-requests", type=int, default=16, help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--serving-workers", type=int, default=4)
+    ap.add_argument("--serving-budget-gb", type=float, default=96.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (counter calibration) and exit")
     args = ap.parse_args()
@@ -413,6 +555,7 @@ def main() -> int:
         log(f"cpu baseline: {cpu}")
 
     lsqr = lsqr_leg(dev, wl, ts, args) if args.lsqr_rays > 0 else None
+    serving = serving_leg(dev, wl, ts, args) if args.serving_requests > 0 else None
     ts.release()
     it_leg = None if args.no_iteration else iteration_leg(dev, cpu=not args.no_cpu)
     uneven = uneven_leg(dev, args) if args.uneven_problems > 0 else None
@@ -450,6 +593,7 @@ def main() -> int:
         "lsqr": lsqr,
         "iteration": it_leg,
         "uneven": uneven,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     dev.close()

@@ -61,6 +61,7 @@ typedef struct ts_tableset ts_tableset;
 typedef struct ts_matrix ts_matrix;
 typedef struct ts_rays ts_rays;
 typedef struct ts_gridbatch ts_gridbatch;
+typedef struct ts_server ts_server;
 
 /* The spherical grid and its metric (SphericalEikonal.cs:15-35 GridMetric): node counts, HLon[Ny*Nz],
  * HLat[Nz], HR, plus the grid bounds the ray tracer needs (GridDefinition, SphericalGrid.cs:13-31). */
@@ -243,6 +244,40 @@ int ts_lsqr_solve(ts_matrix *a, int k, const double *b, const double *damp, int 
                   double btol, double conlim, int mode, const volatile int32_t *cancel, double *x,
                   int32_t *iterations, int32_t *stop, double *stats, double *rnorm_history, char *reason,
                   int reason_len);
+
+/* ---------------------------------------------------------------- serving
+ * A request loop on one GPU (no counterpart in the reference, whose device types serialise their
+ * solves with lock(_gate), EikonalOpenCl.cs:64,165 and LsqrOpenCl.cs:184,257): `workers` threads, each
+ * with its own stream, run table and LSQR requests concurrently; a request waits in a FIFO until its
+ * estimated device memory fits what memory_budget (bytes) leaves; the head of the queue is never
+ * overtaken. Velocity models are resident (ts_server_add_model); LSQR requests name a resident
+ * matrix. A request gives the same result alone and in the mix. */
+int ts_server_create(ts_context *ctx, int workers, uint64_t memory_budget, ts_server **out, char *reason,
+                     int reason_len);
+/* A slowness model resident on the device (FP64 and FP32), kept until the server is destroyed. */
+int ts_server_add_model(ts_server *s, const ts_grid *grid, const double *slowness, int32_t *model_id, char *reason,
+                        int reason_len);
+/* Tables of sources[3*nsources] (lon, lat, depth; point sources, SphericalEikonal.PointSource) in the
+ * P model and, when model_s >= 0, the S model: fields 2s (P) and 2s+1 (S), or s (P only). mode: a
+ * TS_EIK_* mode. The tables stay on the device (and counted against the budget) until the request is
+ * released. */
+int ts_server_submit_tables(ts_server *s, int32_t model_p, int32_t model_s, int nsources, const double *sources,
+                            int mode, int64_t *request_id, char *reason, int reason_len);
+/* ts_lsqr_solve on a resident matrix (b is copied at submission). */
+int ts_server_submit_lsqr(ts_server *s, ts_matrix *a, int k, const double *b, const double *damp, int max_iterations,
+                          double atol, double btol, double conlim, int mode, int64_t *request_id, char *reason,
+                          int reason_len);
+/* Waits up to timeout_s for a request (TS_DECLINED on time-out); status is the request's own status
+ * and times[3] its submission, start and end (seconds since the server started). */
+int ts_server_wait(ts_server *s, int64_t request_id, double timeout_s, int32_t *status, double times[3], char *reason,
+                   int reason_len);
+int ts_server_result_tables(ts_server *s, int64_t request_id, int first, int count, float *out);
+int ts_server_result_lsqr(ts_server *s, int64_t request_id, double *x, int32_t *iterations, int32_t *stop,
+                          double *stats);
+/* Frees a finished request and its device memory. */
+int ts_server_release_request(ts_server *s, int64_t request_id);
+int ts_server_stats(ts_server *s, uint64_t *peak_bytes, int32_t *peak_inflight, uint64_t *budget);
+void ts_server_destroy(ts_server *s);
 
 #ifdef __cplusplus
 }

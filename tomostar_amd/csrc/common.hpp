@@ -32,6 +32,19 @@ struct ts_context {
 
 namespace ts {
 
+// The stream the calling thread's device work runs on: a serving worker runs a request on its own
+// stream whatever context owns the data (a resident matrix, a model), so requests overlap.
+inline hipStream_t &stream_override()
+{
+    static thread_local hipStream_t s = nullptr;
+    return s;
+}
+inline hipStream_t exec_stream(const ts_context *ctx)
+{
+    hipStream_t s = stream_override();
+    return s ? s : ctx->stream;
+}
+
 // A failure inside the library; the ABI wrapper turns it into a status and a reason string.
 struct error : std::runtime_error {
     int status;

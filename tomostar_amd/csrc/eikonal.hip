@@ -879,40 +879,17 @@ int tableset_fields(const ts_tableset *t) { return t->nfields; }
 ts_context *tableset_ctx(const ts_tableset *t) { return t->ctx; }
 }  // namespace ts
 
-// ================================================================================== C ABI
+namespace ts {
 
-extern "C" {
-
-int ts_grid_metric(const ts_grid *grid, double *hlon, double *hlat, double *hr)
-{
-    if (!grid || !hlon || !hlat || !hr || !grid_valid(*grid)) return TS_EINVAL;
-    grid_metric(make_grid(*grid), hlon, hlat, hr);
-    return TS_OK;
-}
-
-int ts_point_source(const ts_grid *grid, const double *slowness, double lon, double lat, double depth_km,
-                    double origin_time, double radius_cells, int32_t *index_out, double *time_out, int cap)
-{
-    if (!grid || !slowness || !grid_valid(*grid) || cap < 0 || (cap > 0 && (!index_out || !time_out)))
-        return TS_EINVAL;
-    if (!std::isfinite(lon) || !std::isfinite(lat) || !std::isfinite(depth_km)) return TS_EINVAL;
-    return point_source(make_grid(*grid), slowness, lon, lat, depth_km, origin_time, radius_cells, index_out,
-                        time_out, cap);
-}
-
-int ts_eik_suggest_batch(ts_context *ctx, int nodes)
-{
-    if (!ctx || nodes <= 0) return TS_EINVAL;
-    return suggest_batch(ctx, nodes);
-}
-
-int ts_tables_create(ts_context *ctx, const ts_grid *grid, int nmodels, const double *const *slowness,
-                     int nfields, const int32_t *model_of_field, const int64_t *seed_offsets,
-                     const int32_t *seed_index, const double *seed_time, ts_tableset **out, char *reason,
-                     int reason_len)
+// ts_tables_create, with the slowness models either on the host (slowness) or resident on the device
+// (dev64 / dev32: FP64 and FP32 copies, the serving path).
+int tables_create_core(ts_context *ctx, const ts_grid *grid, int nmodels, const double *const *slowness,
+                       const double *const *dev64, const float *const *dev32, int nfields,
+                       const int32_t *model_of_field, const int64_t *seed_offsets, const int32_t *seed_index,
+                       const double *seed_time, ts_tableset **out, char *reason, int reason_len)
 {
     if (out) *out = nullptr;
-    if (!ctx || !grid || !out || nmodels < 1 || !slowness || nfields < 1 || !seed_offsets ||
+    if (!ctx || !grid || !out || nmodels < 1 || (!slowness && !(dev64 && dev32)) || nfields < 1 || !seed_offsets ||
         !grid_valid(*grid)) {
         set_reason(reason, reason_len, "invalid argument");
         return TS_EINVAL;
@@ -991,18 +968,26 @@ int ts_tables_create(ts_context *ctx, const ts_grid *grid, int nmodels, const do
         t->hlon.resize((size_t)t->grid.ny * t->grid.nz);
         t->hlat.resize(t->grid.nz);
         grid_metric(t->grid, t->hlon.data(), t->hlat.data(), &t->hr);
-        hipStream_t st = ctx->stream;
+        hipStream_t st = exec_stream(ctx);
         TS_HIP(hipSetDevice(ctx->device));
         t->slow64.alloc((size_t)nmodels * N);
         t->slow32.alloc((size_t)nmodels * N);
         std::vector<float> s32(N);
         for (int m = 0; m < nmodels; m++) {
+            if (dev64 && dev32) {
+                // resident models: device-to-device copies on the solve's stream
+                if (!dev64[m] || !dev32[m]) fail(TS_EINVAL, "resident model %d is null", m);
+                TS_HIP(hipMemcpyAsync(t->slow64.p + (size_t)m * N, dev64[m], (size_t)N * 8, hipMemcpyDeviceToDevice, st));
+                TS_HIP(hipMemcpyAsync(t->slow32.p + (size_t)m * N, dev32[m], (size_t)N * 4, hipMemcpyDeviceToDevice, st));
+                continue;
+            }
             if (!slowness[m]) fail(TS_EINVAL, "slowness model %d is null", m);
             TS_HIP(hipMemcpyAsync(t->slow64.p + (size_t)m * N, slowness[m], (size_t)N * 8, hipMemcpyHostToDevice, st));
             for (int q = 0; q < N; q++) s32[q] = (float)slowness[m][q];
             TS_HIP(hipMemcpyAsync(t->slow32.p + (size_t)m * N, s32.data(), (size_t)N * 4, hipMemcpyHostToDevice, st));
             TS_HIP(hipStreamSynchronize(st));
         }
+        TS_HIP(hipStreamSynchronize(st));
         t->hlon64.alloc(t->hlon.size()); t->hlat64.alloc(t->hlat.size());
         t->hlon32.alloc(t->hlon.size()); t->hlat32.alloc(t->hlat.size());
         std::vector<float> hl32(t->hlon.begin(), t->hlon.end()), ht32(t->hlat.begin(), t->hlat.end());
@@ -1036,6 +1021,50 @@ int ts_tables_create(ts_context *ctx, const ts_grid *grid, int nmodels, const do
         *out = t.release();
         return TS_OK;
     });
+}
+
+
+}  // namespace ts
+
+// ================================================================================== C ABI
+
+extern "C" {
+
+int ts_grid_metric(const ts_grid *grid, double *hlon, double *hlat, double *hr)
+{
+    if (!grid || !hlon || !hlat || !hr || !grid_valid(*grid)) return TS_EINVAL;
+    grid_metric(make_grid(*grid), hlon, hlat, hr);
+    return TS_OK;
+}
+
+int ts_point_source(const ts_grid *grid, const double *slowness, double lon, double lat, double depth_km,
+                    double origin_time, double radius_cells, int32_t *index_out, double *time_out, int cap)
+{
+    if (!grid || !slowness || !grid_valid(*grid) || cap < 0 || (cap > 0 && (!index_out || !time_out)))
+        return TS_EINVAL;
+    if (!std::isfinite(lon) || !std::isfinite(lat) || !std::isfinite(depth_km)) return TS_EINVAL;
+    return point_source(make_grid(*grid), slowness, lon, lat, depth_km, origin_time, radius_cells, index_out,
+                        time_out, cap);
+}
+
+int ts_eik_suggest_batch(ts_context *ctx, int nodes)
+{
+    if (!ctx || nodes <= 0) return TS_EINVAL;
+    return suggest_batch(ctx, nodes);
+}
+
+int ts_tables_create(ts_context *ctx, const ts_grid *grid, int nmodels, const double *const *slowness,
+                     int nfields, const int32_t *model_of_field, const int64_t *seed_offsets,
+                     const int32_t *seed_index, const double *seed_time, ts_tableset **out, char *reason,
+                     int reason_len)
+{
+    if (!slowness) {
+        if (out) *out = nullptr;
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    return ts::tables_create_core(ctx, grid, nmodels, slowness, nullptr, nullptr, nfields, model_of_field,
+                                  seed_offsets, seed_index, seed_time, out, reason, reason_len);
 }
 
 int ts_tables_solve(ts_tableset *t, int mode, const volatile int32_t *cancel, int32_t *rounds_out, char *reason,

@@ -425,7 +425,7 @@ struct SolveOut {
 void lsqr_plain(ts_matrix &A, int k, const double *b, const double *damp, int max_it, double atol, double btol,
                 double conlim, const volatile int32_t *cancel, SolveOut o)
 {
-    hipStream_t st = A.ctx->stream;
+    hipStream_t st = exec_stream(A.ctx);
     const int m = A.rows, n = A.cols;
     if (!A.va64.p) fail(TS_EINVAL, "matrix was created without FP64 values");
     dbuf<double> u(m), v(n), w(n), x(n), tm(m), tn(n), part(64);
@@ -563,7 +563,7 @@ template <int KP, typename T>
 void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int max_it, double atol, double btol,
                   double conlim, const volatile int32_t *cancel, SolveOut o, double *timing)
 {
-    hipStream_t st = A.ctx->stream;
+    hipStream_t st = exec_stream(A.ctx);
     const int m = A.rows, n = A.cols;
     FastLsqr<T> F;
     F.k = k;
@@ -624,7 +624,9 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     TS_HIP(hipStreamSynchronize(st));
     // one iteration as a graph, replayed; the stop flags are read every `check` iterations
     hipGraphExec_t ge = nullptr;
-    if (A.ctx->use_graphs) {
+    // graphs only on the owner's own stream: a serving worker launches directly (captures on several
+    // threads' streams at once are not reliable)
+    if (A.ctx->use_graphs && !stream_override()) {
         hipGraph_t gr;
         TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         fast_iteration<KP, T>(A, F, st, o.hist != nullptr);
@@ -825,6 +827,41 @@ double lsqr_selftest(ts_context *ctx, double *fast_err, char *msg, int len)
 
 }  // namespace ts
 
+namespace ts {
+
+// ts_lsqr_solve without the matrix's gate, on the calling thread's stream (exec_stream): concurrent
+// solves on one resident matrix only read it. seconds: device time of the fast form's iterations.
+int lsqr_solve_core(ts_matrix *a, int k, const double *b, const double *damp, int max_iterations, double atol,
+                    double btol, double conlim, int mode, const volatile int32_t *cancel, double *x,
+                    int32_t *iterations, int32_t *stop, double *stats, double *rnorm_history, double *seconds,
+                    char *reason, int reason_len)
+{
+    if (!a || k < 1 || k > 64 || !b || !damp || max_iterations < 0 || !x || !iterations || !stop || !stats ||
+        (mode != TS_LSQR_PLAIN_FP64 && mode != TS_LSQR_FAST_FP32 && mode != TS_LSQR_FAST_MIXED)) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    return guarded(reason, reason_len, [&]() -> int {
+        TS_HIP(hipSetDevice(a->ctx->device));
+        if (a->rows == 0 || a->cols == 0) {
+            for (int j = 0; j < k; j++) {
+                iterations[j] = 0;
+                stop[j] = 8;  // empty system (Lsqr.cs:48)
+                stats[3 * j] = stats[3 * j + 1] = stats[3 * j + 2] = 0;
+            }
+            std::fill(x, x + (size_t)k * a->cols, 0.0);
+            return TS_OK;
+        }
+        SolveOut o{x, iterations, stop, stats, rnorm_history};
+        if (mode == TS_LSQR_PLAIN_FP64) lsqr_plain(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o);
+        else lsqr_fast(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o, seconds,
+                       mode == TS_LSQR_FAST_MIXED);
+        return TS_OK;
+    });
+}
+
+}  // namespace ts
+
 // ================================================================================== C ABI
 
 extern "C" {
@@ -888,29 +925,16 @@ int ts_lsqr_solve(ts_matrix *a, int k, const double *b, const double *damp, int 
                   int32_t *iterations, int32_t *stop, double *stats, double *rnorm_history, char *reason,
                   int reason_len)
 {
-    if (!a || k < 1 || k > 64 || !b || !damp || max_iterations < 0 || !x || !iterations || !stop || !stats ||
-        (mode != TS_LSQR_PLAIN_FP64 && mode != TS_LSQR_FAST_FP32 && mode != TS_LSQR_FAST_MIXED)) {
+    if (!a) {
         set_reason(reason, reason_len, "invalid argument");
         return TS_EINVAL;
     }
     std::lock_guard<std::mutex> lk(a->gate);
-    return guarded(reason, reason_len, [&]() -> int {
-        TS_HIP(hipSetDevice(a->ctx->device));
-        if (a->rows == 0 || a->cols == 0) {
-            for (int j = 0; j < k; j++) {
-                iterations[j] = 0;
-                stop[j] = 8;  // empty system (Lsqr.cs:48)
-                stats[3 * j] = stats[3 * j + 1] = stats[3 * j + 2] = 0;
-            }
-            std::fill(x, x + (size_t)k * a->cols, 0.0);
-            return TS_OK;
-        }
-        SolveOut o{x, iterations, stop, stats, rnorm_history};
-        if (mode == TS_LSQR_PLAIN_FP64) lsqr_plain(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o);
-        else lsqr_fast(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o, &a->last_seconds,
-                       mode == TS_LSQR_FAST_MIXED);
-        return TS_OK;
-    });
+    double seconds = 0;
+    const int rc = ts::lsqr_solve_core(a, k, b, damp, max_iterations, atol, btol, conlim, mode, cancel, x, iterations,
+                                       stop, stats, rnorm_history, &seconds, reason, reason_len);
+    if (rc == TS_OK) a->last_seconds = seconds;
+    return rc;
 }
 
 }  // extern "C"

@@ -30,4 +30,10 @@ namespace ts {
 // SparseMatrix.cs:79-97 does. d_va64 is adopted as va64 when keep_fp64, released otherwise.
 void matrix_finish(ts_matrix &A, dbuf<double> &&d_va64, bool keep_fp64);
 
+// ts_lsqr_solve without the matrix's gate, on the calling thread's stream.
+int lsqr_solve_core(ts_matrix *a, int k, const double *b, const double *damp, int max_iterations, double atol,
+                    double btol, double conlim, int mode, const volatile int32_t *cancel, double *x,
+                    int32_t *iterations, int32_t *stop, double *stats, double *rnorm_history, double *seconds,
+                    char *reason, int reason_len);
+
 }  // namespace ts
