@@ -83,16 +83,24 @@ def test_plain_stop_rules_follow_the_oracle(device, oracle):
         assert rc == 0, why
         x, stop, st, hist = _oracle(oracle, m, n, rs, ci, va, b, damp, 300, atol=atol, btol=atol, conlim=conlim)
         r = res[0]
-        assert (r.iterations, r.stop_code) == (st["iterations"], stop)
+        # the same stop rule fires; a threshold crossed within rounding may be crossed one iteration
+        # apart, as the device stores the rows in another order and reduces in another order
+        assert r.stop_code == stop and abs(r.iterations - st["iterations"]) <= 1
         # dot products reduce in a different order on the device (64 partials) and on the host (SIMD
         # lanes): once the Golub-Kahan vectors lose orthogonality the two runs follow different (equally
         # valid) paths whose residual estimates part by up to ~1e-6 before both converge
-        assert np.allclose(r.residual_history, hist, rtol=1e-5, atol=0)
+        common = min(r.iterations, st["iterations"])
+        assert np.allclose(r.residual_history[:common], hist[:common], rtol=1e-5, atol=0)
         # the two paths meet at the same solution to the accuracy the stop rule asked for, or to the
         # spread rounding alone causes: an early stop (conlim 50: 20 iterations) leaves an iterate that
-        # moves by 1e-3 when b is perturbed by 1e-15 - measured here on the oracle itself
+        # moves by 1e-3 when b is perturbed by 1e-15 - measured here on the oracle itself (both at the
+        # device's iteration count when the counts differ)
         bp = b * (1 + 1e-15 * np.random.default_rng(3).standard_normal(b.size))
-        xp = _oracle(oracle, m, n, rs, ci, va, bp, damp, 300, atol=atol, btol=atol, conlim=conlim)[0]
+        if r.iterations != st["iterations"]:
+            x = _oracle(oracle, m, n, rs, ci, va, b, damp, r.iterations, atol=0, btol=0, conlim=0)[0]
+            xp = _oracle(oracle, m, n, rs, ci, va, bp, damp, r.iterations, atol=0, btol=0, conlim=0)[0]
+        else:
+            xp = _oracle(oracle, m, n, rs, ci, va, bp, damp, 300, atol=atol, btol=atol, conlim=conlim)[0]
         spread = np.linalg.norm(xp - x) / np.linalg.norm(x)
         assert np.linalg.norm(r.solution - x) <= max(1e-7, 10 * atol, 20 * spread) * np.linalg.norm(x)
 

@@ -16,6 +16,7 @@ struct ts_matrix {
     ts::dbuf<int32_t> rs, ci, trs, tci;  // A and its transpose
     ts::dbuf<float> va32, tva32;
     ts::dbuf<double> va64, tva64;        // kept only when the plain FP64 form is enabled
+    ts::dbuf<int32_t> row_perm;          // stored row r is row row_perm[r] of the matrix as given
     std::vector<double> rhs;         // right-hand side of an assembled system (empty otherwise)
     double last_seconds = 0;         // device time of the last fast solve's iterations (HIP events)
     double build_seconds = 0;        // wall time of the assembly / upload and transpose

@@ -573,19 +573,39 @@ int ts_matrix_read(ts_matrix *a, int32_t *row_start, int32_t *column, double *va
     return guarded(nullptr, 0, [&]() -> int {
         TS_HIP(hipSetDevice(a->ctx->device));
         std::lock_guard<std::mutex> lk(a->gate);
-        if (row_start) TS_HIP(hipMemcpy(row_start, a->rs.p, ((size_t)a->rows + 1) * 4, hipMemcpyDeviceToHost));
-        if (a->nnz > 0 && column) TS_HIP(hipMemcpy(column, a->ci.p, a->nnz * 4, hipMemcpyDeviceToHost));
-        if (a->nnz > 0 && values) {
-            if (a->va64.p) TS_HIP(hipMemcpy(values, a->va64.p, a->nnz * 8, hipMemcpyDeviceToHost));
-            else {
-                std::vector<float> f(a->nnz);
-                TS_HIP(hipMemcpy(f.data(), a->va32.p, a->nnz * 4, hipMemcpyDeviceToHost));
-                for (int64_t q = 0; q < a->nnz; q++) values[q] = f[q];
-            }
-        }
+        const int m = a->rows;
+        const int64_t nnz = a->nnz;
         if (rhs) {
             if (a->rhs.size() != (size_t)a->rows) fail(TS_EINVAL, "the matrix was not assembled here: no right-hand side");
             std::copy(a->rhs.begin(), a->rhs.end(), rhs);
+        }
+        if (!row_start && !column && !values) return TS_OK;
+        // the stored arrays, then the rows back in the order they were given
+        std::vector<int32_t> rs(m + 1), ci(std::max<int64_t>(1, nnz)), perm(std::max(1, m));
+        std::vector<double> va(std::max<int64_t>(1, nnz));
+        TS_HIP(hipMemcpy(rs.data(), a->rs.p, ((size_t)m + 1) * 4, hipMemcpyDeviceToHost));
+        if (nnz > 0) {
+            TS_HIP(hipMemcpy(ci.data(), a->ci.p, nnz * 4, hipMemcpyDeviceToHost));
+            if (a->va64.p) TS_HIP(hipMemcpy(va.data(), a->va64.p, nnz * 8, hipMemcpyDeviceToHost));
+            else {
+                std::vector<float> f(nnz);
+                TS_HIP(hipMemcpy(f.data(), a->va32.p, nnz * 4, hipMemcpyDeviceToHost));
+                for (int64_t q = 0; q < nnz; q++) va[q] = f[q];
+            }
+        }
+        if (a->row_perm.p) TS_HIP(hipMemcpy(perm.data(), a->row_perm.p, (size_t)m * 4, hipMemcpyDeviceToHost));
+        else for (int r = 0; r < m; r++) perm[r] = r;
+        std::vector<int32_t> where(std::max(1, m));  // stored position of each given row
+        for (int r = 0; r < m; r++) where[perm[r]] = r;
+        int64_t at = 0;
+        if (row_start) row_start[0] = 0;
+        for (int o = 0; o < m; o++) {
+            const int r = where[o];
+            for (int32_t q = rs[r]; q < rs[r + 1]; q++, at++) {
+                if (column) column[at] = ci[q];
+                if (values) values[at] = va[q];
+            }
+            if (row_start) row_start[o + 1] = (int32_t)at;
         }
         return TS_OK;
     });
@@ -597,9 +617,29 @@ int ts_matrix_read_transpose(ts_matrix *a, int32_t *row_start, int32_t *column, 
     return guarded(nullptr, 0, [&]() -> int {
         TS_HIP(hipSetDevice(a->ctx->device));
         std::lock_guard<std::mutex> lk(a->gate);
-        if (row_start) TS_HIP(hipMemcpy(row_start, a->trs.p, ((size_t)a->cols + 1) * 4, hipMemcpyDeviceToHost));
-        if (a->nnz > 0 && column) TS_HIP(hipMemcpy(column, a->tci.p, a->nnz * 4, hipMemcpyDeviceToHost));
-        if (a->nnz > 0 && values) TS_HIP(hipMemcpy(values, a->tva32.p, a->nnz * 4, hipMemcpyDeviceToHost));
+        const int n = a->cols, m = a->rows;
+        const int64_t nnz = a->nnz;
+        std::vector<int32_t> trs(n + 1), tci(std::max<int64_t>(1, nnz)), perm(std::max(1, m));
+        std::vector<float> tva(std::max<int64_t>(1, nnz));
+        TS_HIP(hipMemcpy(trs.data(), a->trs.p, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+        if (nnz > 0) {
+            TS_HIP(hipMemcpy(tci.data(), a->tci.p, nnz * 4, hipMemcpyDeviceToHost));
+            TS_HIP(hipMemcpy(tva.data(), a->tva32.p, nnz * 4, hipMemcpyDeviceToHost));
+        }
+        if (a->row_perm.p && m > 0) {
+            // stored row indices back to the given ones, each transposed row in increasing order again
+            TS_HIP(hipMemcpy(perm.data(), a->row_perm.p, (size_t)m * 4, hipMemcpyDeviceToHost));
+            std::vector<std::pair<int32_t, float>> e;
+            for (int c = 0; c < n; c++) {
+                e.clear();
+                for (int32_t q = trs[c]; q < trs[c + 1]; q++) e.emplace_back(perm[tci[q]], tva[q]);
+                std::sort(e.begin(), e.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+                for (size_t i = 0; i < e.size(); i++) { tci[trs[c] + i] = e[i].first; tva[trs[c] + i] = e[i].second; }
+            }
+        }
+        if (row_start) std::copy(trs.begin(), trs.end(), row_start);
+        if (column && nnz > 0) std::copy(tci.begin(), tci.begin() + nnz, column);
+        if (values && nnz > 0) std::copy(tva.begin(), tva.begin() + nnz, values);
         return TS_OK;
     });
 }
