@@ -332,8 +332,13 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int di
     const int g = g0 + blockIdx.y;
     const uint64_t act = A.active[g];
     if (act == 0) return;
-    const int stride = gridDim.x * WPB;
-    for (int a = blockIdx.x * WPB + (threadIdx.x >> 6); a < cnt; a += stride)
+    // Workgroups go to the eight XCDs in turn (block b to XCD b mod 8); block b takes the b/8-th slice
+    // of XCD (b mod 8)'s contiguous share of the plane, so the nodes one XCD updates are neighbours and
+    // the rows their stencils share meet in that XCD's L2.
+    const int per_xcd = (gridDim.x + 7) / 8;
+    const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int stride = 8 * per_xcd * WPB;
+    for (int a = lb * WPB + (threadIdx.x >> 6); a < cnt; a += stride)
         update_node<R, kCount, kOff32, sizeof(R) == 4>(A, dir, A.planes[off + a], g, act, lane, blockIdx.x);
 }
 
@@ -705,7 +710,9 @@ void launch_round(ts_tableset &t, R *T, hipStream_t st, int hold)
         for (int dir = 0; dir < 8; dir++)
             for (int p = 0; p < P; p++) {
                 int off = t.plane_off[p], cnt = t.plane_off[p + 1] - off;
-                dim3 grid(std::min<unsigned>(blocks_for(cnt, kWPB), std::max(1, plane_blocks(t) / gc)), gc);
+                // a multiple of 8 blocks: each XCD gets the same number of slices
+                const unsigned nb = std::min<unsigned>(blocks_for(cnt, kWPB), std::max(1, plane_blocks(t) / gc));
+                dim3 grid((nb + 7) / 8 * 8, gc);
                 if (off32) eik_fast_plane<R, kWPB, kCount, true><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
                 else eik_fast_plane<R, kWPB, kCount, false><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
             }
