@@ -305,11 +305,14 @@ __device__ inline void update_node(const FastArgs<R> &A, int dir, uint32_t pk, i
     }
     const uint64_t woke = __ballot(dropped);
     if (woke) {
-        // Wake the twelve nodes that read this one (one, two steps on each axis), lanes 0-11.
-        if (lane < 12) {
-            const int ax = lane >> 2, which = lane & 3;
+        // Wake the nodes that read this one: one and two steps on each axis (lanes 0-11) when a field of
+        // the group is in its second-order phase, else one step (lanes 0-5) - a first-order update reads
+        // no node two steps away, and a field entering its second-order phase wakes every node.
+        if (lane < (any_second ? 12 : 6)) {
+            const int ax = any_second ? lane >> 2 : lane >> 1;
             const int pos = ax == 0 ? i : ax == 1 ? j : k, n = ax == 0 ? nx : ax == 1 ? ny : nz;
             const int st = ax == 0 ? 1 : ax == 1 ? nx : nxy;
+            const int which = any_second ? lane & 3 : (lane & 1) * 2;
             const int d = which == 0 ? -1 : which == 1 ? -2 : which == 2 ? 1 : 2;
             const int np = pos + d;
             if (np >= 0 && np < n) A.dirty[gn + node + d * st] = 1;
@@ -410,24 +413,34 @@ __global__ __launch_bounds__(64) void eik_mixed_promote(uint64_t *active, uint64
     }
 }
 
+// Every time unreached and every node clean: an update of a node whose neighbours are all unreached
+// returns +inf and changes nothing, so only the readers of the seeds need a first visit.
 template <typename R>
 __global__ void eik_fast_clear(R *T, uint8_t *dirty, uint64_t *seedmask, size_t nodes_total)
 {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < nodes_total * kLanes) T[q] = (R)INFINITY;
-    if (q < nodes_total) { dirty[q] = 1; seedmask[q] = 0; }
+    if (q < nodes_total) { dirty[q] = 0; seedmask[q] = 0; }
 }
 
+// Seed times, seed masks and, when wake is set, the dirty flags of the twelve nodes that read each
+// seed (one and two steps on each axis).
 template <typename R>
-__global__ void eik_fast_seed(R *T, uint64_t *seedmask, const int32_t *field, const int32_t *node,
-                              const double *time, int count, int N)
+__global__ void eik_fast_seed(R *T, uint64_t *seedmask, uint8_t *dirty, const int32_t *field, const int32_t *node,
+                              const double *time, int count, int nx, int ny, int nz, bool wake)
 {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
-    const int f = field[q], g = f / kLanes, lane = f % kLanes;
-    const size_t at = (size_t)g * N + node[q];
+    const int N = nx * ny * nz;
+    const int f = field[q], g = f / kLanes, lane = f % kLanes, n = node[q];
+    const size_t at = (size_t)g * N + n;
     T[at * kLanes + lane] = (R)time[q];
     atomicOr((unsigned long long *)&seedmask[at], 1ull << lane);
+    if (!wake) return;
+    const int pos[3] = {n % nx, (n / nx) % ny, n / (nx * ny)}, ext[3] = {nx, ny, nz}, st[3] = {1, nx, nx * ny};
+    for (int ax = 0; ax < 3; ax++)
+        for (int d = -2; d <= 2; d++)
+            if (d != 0 && pos[ax] + d >= 0 && pos[ax] + d < ext[ax]) dirty[at + d * st[ax]] = 1;
 }
 
 // [group][node][64] -> field-major tables (FP32, and FP64 when requested), through an LDS tile so
@@ -742,13 +755,13 @@ void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count,
     }
 }
 
-// The first round of a phase that reads the dirty flag before the other loads: FP32 times from the
-// start, FP64 times from round TOMOSTAR_FP64_FLAG_FIRST (default 1): every node is dirty when a phase
-// starts, and in the mixed form's second-order phase most are clean after its first round. In the FP64
-// form rounding-level drops keep most nodes dirty and both load orders take the same time.
-int flag_first_round(size_t width)
+// The first round of a run that reads the dirty flag before the other loads. A solve starts with only
+// the seeds' readers dirty, so from round 0; the mixed form's second-order phase starts with every node
+// dirty and most of them clean after its first round, so from round 1 (TOMOSTAR_FP64_FLAG_FIRST
+// overrides).
+int flag_first_round(bool all_dirty)
 {
-    if (width == 4) return 0;
+    if (!all_dirty) return 0;
     if (const char *e = getenv("TOMOSTAR_FP64_FLAG_FIRST")) return std::max(0, atoi(e));
     return 1;
 }
@@ -783,9 +796,9 @@ hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count, bool early)
 // Rounds of one graph until no field is relaxing; the sweep time is bracketed by HIP events on the
 // solve's stream. Declines when a field exhausts its round cap.
 template <typename R>
-void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel, int32_t *h_status)
+void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel, int32_t *h_status, bool all_dirty)
 {
-    const int hold = sizeof(R) == 4, early_from = flag_first_round(sizeof(R));
+    const int hold = sizeof(R) == 4, early_from = flag_first_round(all_dirty);
     hipStream_t st = t.ctx->stream;
     hipEvent_t ev0, ev1;
     TS_HIP(hipEventCreate(&ev0));
@@ -879,10 +892,12 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     const unsigned cb = blocks_for(nodes_total * kLanes, 256), sb = blocks_for(sf.size(), 256);
     if (mixed) {
         eik_fast_clear<float><<<cb, 256, 0, st>>>(t.T32.p, t.dirty.p, t.seedmask.p, nodes_total);
-        eik_fast_seed<float><<<sb, 256, 0, st>>>(t.T32.p, t.seedmask.p, df.p, dn.p, dv.p, (int)sf.size(), N);
+        eik_fast_seed<float><<<sb, 256, 0, st>>>(t.T32.p, t.seedmask.p, t.dirty.p, df.p, dn.p, dv.p, (int)sf.size(),
+                                                 t.grid.nx, t.grid.ny, t.grid.nz, true);
     } else {
         eik_fast_clear<double><<<cb, 256, 0, st>>>(t.T64.p, t.dirty.p, t.seedmask.p, nodes_total);
-        eik_fast_seed<double><<<sb, 256, 0, st>>>(t.T64.p, t.seedmask.p, df.p, dn.p, dv.p, (int)sf.size(), N);
+        eik_fast_seed<double><<<sb, 256, 0, st>>>(t.T64.p, t.seedmask.p, t.dirty.p, df.p, dn.p, dv.p, (int)sf.size(),
+                                                  t.grid.nx, t.grid.ny, t.grid.nz, true);
     }
     TS_HIP(hipGetLastError());
     TS_HIP(hipStreamSynchronize(st));
@@ -897,15 +912,16 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     t.wake_tol[0] = 0.0;
     t.wake_tol[1] = mixed ? mixed_wake_tol() : 0.0;
     if (mixed) {
-        run_rounds<float>(t, t.T32.p, count, cancel, h_status);
+        run_rounds<float>(t, t.T32.p, count, cancel, h_status, false);
         eik_mixed_widen<<<blocks_for(nodes_total * kLanes, 256), 256, 0, st>>>(t.T32.p, t.T64.p, nodes_total * kLanes);
         // the fixed seed times again, unrounded
-        eik_fast_seed<double><<<sb, 256, 0, st>>>(t.T64.p, t.seedmask.p, df.p, dn.p, dv.p, (int)sf.size(), N);
+        eik_fast_seed<double><<<sb, 256, 0, st>>>(t.T64.p, t.seedmask.p, t.dirty.p, df.p, dn.p, dv.p, (int)sf.size(),
+                                                  t.grid.nx, t.grid.ny, t.grid.nz, false);
         eik_mixed_promote<<<groups, 64, 0, st>>>(t.active.p, t.phase1.p, t.changed.p, t.state.p);
         eik_fill_u8<<<blocks_for(nodes_total, 256), 256, 0, st>>>(t.dirty.p, nodes_total, 1);
         TS_HIP(hipGetLastError());
     }
-    run_rounds<double>(t, t.T64.p, count, cancel, h_status);
+    run_rounds<double>(t, t.T64.p, count, cancel, h_status, mixed);
     dim3 ug(blocks_for(N, 64), groups);
     eik_fast_unpack<double><<<ug, 256, 0, st>>>(t.T64.p, N, t.field_of_slot_dev.p, t.tab32.p, t.tab64.p);
     TS_HIP(hipGetLastError());
