@@ -231,26 +231,37 @@ __device__ inline T at_off(const T *base, uint32_t byte_off)
     return *(const T *)((const char *)base + byte_off);
 }
 
-// The update of plane node a for the 64 fields of group g (one wavefront, one lane per field). With
-// kEarly the dirty flag is read first and a clean node costs one load; otherwise every load of the
-// update - the flag, the seed mask, the slowness, the node and its first and second neighbours on
-// each axis - is issued at once, so a wave waits for memory once. No node of a plane reads another
-// node of the same plane (neighbours lie one or two planes away), so reading early reads the same
-// values.
-template <typename R, bool kCount, bool kOff32, bool kEarly>
+// How an update treats the node's dirty flag: read it with every other load of the update (the seed
+// mask, the slowness, the node and its first and second neighbours on each axis), so a wave waits for
+// memory once; read it first, so a clean node costs one load; or know it is set (checked beforehand).
+enum FlagRead { kFlagWithLoads = 0, kFlagFirst = 1, kFlagKnown = 2 };
+
+// Grid node of packed plane entry pk (i' | j' << 11 | k' << 22 in the ordering's mirrored coordinates).
+template <typename R>
+__device__ inline int plane_node(const FastArgs<R> &A, int dir, uint32_t pk)
+{
+    const int ip = pk & 2047, jp = (pk >> 11) & 2047, kp = pk >> 22;
+    const int i = (dir & 1) ? A.nx - 1 - ip : ip;
+    const int j = (dir & 2) ? A.ny - 1 - jp : jp;
+    const int k = (dir & 4) ? A.nz - 1 - kp : kp;
+    return (k * A.ny + j) * A.nx + i;
+}
+
+// The update of plane node pk for the 64 fields of group g (one wavefront, one lane per field). No
+// node of a plane reads another node of the same plane (neighbours lie one or two planes away), so
+// the order of the loads does not change what they read.
+template <typename R, bool kCount, bool kOff32, int kFlag>
 __device__ inline void update_node(const FastArgs<R> &A, int dir, uint32_t pk, int g, uint64_t act, int lane,
                                    unsigned counter)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
-    const int ip = pk & 2047, jp = (pk >> 11) & 2047, kp = pk >> 22;
-    const int i = (dir & 1) ? nx - 1 - ip : ip;
-    const int j = (dir & 2) ? ny - 1 - jp : jp;
-    const int k = (dir & 4) ? nz - 1 - kp : kp;
-    const int node = (k * ny + j) * nx + i;
+    const int node = plane_node(A, dir, pk);
+    const int i = node % nx, j = (node / nx) % ny, k = node / nxy;
     const size_t gn = (size_t)g * A.N;
     uint8_t *flag = A.dirty + gn + node;
-    const uint8_t fl = *flag;
-    if (kEarly && fl == 0) return;  // nothing this node reads has moved: the update would return its value
+    uint8_t fl = 1;
+    if constexpr (kFlag != kFlagKnown) fl = *flag;
+    if (kFlag == kFlagFirst && fl == 0) return;  // nothing this node reads has moved: the update would return its value
     const uint64_t fixed = A.seedmask[gn + node];
     const uint64_t ph = A.phase1[g];
     const bool any_second = (ph & act) != 0;  // wave-uniform
@@ -327,10 +338,9 @@ __device__ inline void update_node(const FastArgs<R> &A, int dir, uint32_t pk, i
 
 // One diagonal plane of one sweep ordering for the groups g0 .. g0 + gridDim.y - 1: each wavefront
 // updates plane nodes a, a + W, a + 2W, ... (W the waves of a group's row of blocks), so a launch
-// is a few resident waves per SIMD rather than one short-lived workgroup per node. With kEarly the
-// dirty flag is read first (rounds where most nodes are clean), otherwise every load is issued at
-// once (rounds where most are not).
-template <typename R, int WPB, bool kCount, bool kOff32, bool kEarly>
+// is a few resident waves per SIMD rather than one short-lived workgroup per node. kFlag: the dirty
+// flag read first (rounds where most nodes are clean) or with every other load (where most are not).
+template <typename R, int WPB, bool kCount, bool kOff32, int kFlag>
 __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
@@ -344,7 +354,33 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int di
     const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     const int stride = 8 * per_xcd * WPB;
     for (int a = lb * WPB + (threadIdx.x >> 6); a < cnt; a += stride)
-        update_node<R, kCount, kOff32, kEarly>(A, dir, A.planes[off + a], g, act, lane, blockIdx.x);
+        update_node<R, kCount, kOff32, kFlag>(A, dir, A.planes[off + a], g, act, lane, blockIdx.x);
+}
+
+// Rounds in which most nodes are clean: each wavefront reads the dirty flags of M consecutive plane
+// nodes at once (lanes 0 .. M-1) and then updates its dirty ones in plane order, so a clean stretch of
+// the plane costs one load per M nodes and M times fewer waves are launched.
+template <typename R, int WPB, int M, bool kCount, bool kOff32>
+__global__ __launch_bounds__(64 * WPB) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
+{
+    const int lane = threadIdx.x & 63;
+    const int g = g0 + blockIdx.y;
+    const uint64_t act = A.active[g];
+    if (act == 0) return;
+    const int per_xcd = (gridDim.x + 7) / 8;
+    const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int a0 = (lb * WPB + (threadIdx.x >> 6)) * M;
+    if (a0 >= cnt) return;
+    uint32_t pk = 0;
+    bool dirty = false;
+    if (lane < M && a0 + lane < cnt) {
+        pk = A.planes[off + a0 + lane];
+        dirty = A.dirty[(size_t)g * A.N + plane_node(A, dir, pk)] != 0;
+    }
+    for (uint64_t todo = __ballot(dirty); todo; todo &= todo - 1) {
+        const uint32_t p = __shfl(pk, __builtin_ctzll(todo));
+        update_node<R, kCount, kOff32, kFlagKnown>(A, dir, p, g, act, lane, blockIdx.x);
+    }
 }
 
 // Per-field bookkeeping after a round (one wave per group): a field whose round moved no node by
@@ -519,8 +555,8 @@ struct ts_tableset {
     dbuf<int32_t> model_dev, state, rounds_dev, status;
     dbuf<unsigned long long> evals;
     double wake_tol[2] = {0, 0};          // FastArgs::wake_tol of the running solve's FP32 / FP64 sweeps
-    hipGraphExec_t graphs[2][2][2][2] = {};  // [FP64][counting][dirty flag first][wake_tol > 0]
-    double graph_wake_tol[2][2][2][2] = {};  // the wake_tol each graph was captured with
+    hipGraphExec_t graphs[2][2][3][2] = {};  // [FP64][counting][flag reads][wake_tol > 0]
+    double graph_wake_tol[2][2][3][2] = {};  // the wake_tol each graph was captured with
     std::mutex gate;
     ~ts_tableset()
     {
@@ -693,6 +729,11 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
 }
 
 constexpr int kWPB = 4;
+constexpr int kSparse = 2;  // round order after kFlagWithLoads, kFlagFirst: kSparseNodes flags per wave, then the dirty nodes
+#ifndef TS_SPARSE_NODES
+#define TS_SPARSE_NODES 4
+#endif
+constexpr int kSparseNodes = TS_SPARSE_NODES;
 
 // Cap on the blocks of a plane launch over all its groups (TOMOSTAR_PLANE_BLOCKS): waves stride over
 // the plane beyond it. Measured on the 400-field case: fewer, longer-lived waves are slower (the
@@ -712,7 +753,7 @@ int group_chunk(const ts_tableset &t, size_t width)
     return (t.nfields + kLanes - 1) / kLanes;
 }
 
-template <typename R, bool kCount, bool kEarly>
+template <typename R, bool kCount, int kOrder>
 void launch_round(ts_tableset &t, R *T, hipStream_t st, int hold)
 {
     const int groups = (t.nfields + kLanes - 1) / kLanes;
@@ -730,11 +771,20 @@ void launch_round(ts_tableset &t, R *T, hipStream_t st, int hold)
         for (int dir = 0; dir < 8; dir++)
             for (int p = 0; p < P; p++) {
                 int off = t.plane_off[p], cnt = t.plane_off[p + 1] - off;
-                // a multiple of 8 blocks: each XCD gets the same number of slices
-                const unsigned nb = std::min<unsigned>(blocks_for(cnt, kWPB), std::max(1, plane_blocks(t) / gc));
-                dim3 grid((nb + 7) / 8 * 8, gc);
-                if (off32) eik_fast_plane<R, kWPB, kCount, true, kEarly><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
-                else eik_fast_plane<R, kWPB, kCount, false, kEarly><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                if constexpr (kOrder == kSparse) {
+                    const unsigned nb = blocks_for(cnt, kWPB * kSparseNodes);
+                    dim3 grid((nb + 7) / 8 * 8, gc);
+                    if (off32)
+                        eik_fast_plane_sparse<R, kWPB, kSparseNodes, kCount, true><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                    else
+                        eik_fast_plane_sparse<R, kWPB, kSparseNodes, kCount, false><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                } else {
+                    // a multiple of 8 blocks: each XCD gets the same number of slices
+                    const unsigned nb = std::min<unsigned>(blocks_for(cnt, kWPB), std::max(1, plane_blocks(t) / gc));
+                    dim3 grid((nb + 7) / 8 * 8, gc);
+                    if (off32) eik_fast_plane<R, kWPB, kCount, true, kOrder><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                    else eik_fast_plane<R, kWPB, kCount, false, kOrder><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                }
             }
     }
     eik_fast_round_end<<<groups, 64, 0, st>>>(t.active.p, t.phase1.p, t.changed.p, t.state.p, t.rounds_dev.p,
@@ -743,38 +793,50 @@ void launch_round(ts_tableset &t, R *T, hipStream_t st, int hold)
     eik_fast_reset_dirty<<<rg, 256, 0, st>>>(t.dirty.p, t.reset.p, t.N);
 }
 
-template <typename R>
-void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count, bool early)
+template <typename R, bool kCount>
+void launch_round_order(ts_tableset &t, R *T, hipStream_t st, int hold, int order)
 {
-    if (count) {
-        if (early) launch_round<R, true, true>(t, T, st, hold);
-        else launch_round<R, true, false>(t, T, st, hold);
-    } else {
-        if (early) launch_round<R, false, true>(t, T, st, hold);
-        else launch_round<R, false, false>(t, T, st, hold);
-    }
+    if (order == kFlagWithLoads) launch_round<R, kCount, kFlagWithLoads>(t, T, st, hold);
+    else if (order == kFlagFirst) launch_round<R, kCount, kFlagFirst>(t, T, st, hold);
+    else launch_round<R, kCount, kSparse>(t, T, st, hold);
 }
 
-// The first round of a run that reads the dirty flag before the other loads. A solve starts with only
-// the seeds' readers dirty, so from round 0; the mixed form's second-order phase starts with every node
-// dirty and most of them clean after its first round, so from round 1 (TOMOSTAR_FP64_FLAG_FIRST
-// overrides).
-int flag_first_round(bool all_dirty)
+template <typename R>
+void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count, int order)
 {
-    if (!all_dirty) return 0;
-    if (const char *e = getenv("TOMOSTAR_FP64_FLAG_FIRST")) return std::max(0, atoi(e));
-    return 1;
+    if (count) launch_round_order<R, true>(t, T, st, hold, order);
+    else launch_round_order<R, false>(t, T, st, hold, order);
+}
+
+// How round `round` of a run reads the dirty flags. A solve starts with only the seeds' readers dirty,
+// so flags first from round 0; the mixed form's second-order phase starts with every node dirty and
+// most of them clean after its first round, so from round 1 (TOMOSTAR_FP64_FLAG_FIRST overrides). Two
+// rounds later, when the sweeps mostly confirm, a wave reads kSparseNodes flags at once
+// (TOMOSTAR_SPARSE_FROM overrides) - in runs of one phase: in the FP64 form a field's second-order
+// phase starts in mid-run with every node of its group dirty again. Measured on 400 fields of 201 x 201 x 101: 4 flags per wave from
+// these rounds on take the solve from 1.26 s to 1.20 s; 16 or 32 flags per wave, or sparse rounds
+// starting earlier, are slower (fewer waves left to hide the latency of the dirty nodes).
+int round_order(bool all_dirty, bool one_phase, int round)
+{
+    int first = 0;
+    if (all_dirty) {
+        first = 1;
+        if (const char *e = getenv("TOMOSTAR_FP64_FLAG_FIRST")) first = std::max(0, atoi(e));
+    }
+    int sparse = first + 2;
+    if (const char *e = getenv("TOMOSTAR_SPARSE_FROM")) sparse = std::max(0, atoi(e));
+    return round < first ? kFlagWithLoads : round < sparse || !one_phase ? kFlagFirst : kSparse;
 }
 
 // One round as a graph: 8 orderings x all planes x all groups, then the per-field bookkeeping. The
 // FP32 graph holds fields at the end of their first-order phase (mixed form), the FP64 one does not.
 template <typename R>
-hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count, bool early)
+hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count, int order)
 {
     const int w = sizeof(R) == 8;
     const int woken = t.wake_tol[w] > 0;
-    hipGraphExec_t &ge = t.graphs[w][count][early][woken];
-    double &captured = t.graph_wake_tol[w][count][early][woken];
+    hipGraphExec_t &ge = t.graphs[w][count][order][woken];
+    double &captured = t.graph_wake_tol[w][count][order][woken];
     if (ge && captured != t.wake_tol[w]) {  // captured with another tolerance
         TS_HIP(hipGraphExecDestroy(ge));
         ge = nullptr;
@@ -785,7 +847,7 @@ hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count, bool early)
     hipGraph_t gr;
     TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const int hold = sizeof(R) == 4;
-    launch_round_as<R>(t, T, st, hold, count, early);
+    launch_round_as<R>(t, T, st, hold, count, order);
     TS_HIP(hipStreamEndCapture(st, &gr));
     hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
     (void)hipGraphDestroy(gr);
@@ -796,9 +858,10 @@ hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count, bool early)
 // Rounds of one graph until no field is relaxing; the sweep time is bracketed by HIP events on the
 // solve's stream. Declines when a field exhausts its round cap.
 template <typename R>
-void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel, int32_t *h_status, bool all_dirty)
+void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel, int32_t *h_status, bool all_dirty,
+                bool one_phase)
 {
-    const int hold = sizeof(R) == 4, early_from = flag_first_round(all_dirty);
+    const int hold = sizeof(R) == 4;
     hipStream_t st = t.ctx->stream;
     hipEvent_t ev0, ev1;
     TS_HIP(hipEventCreate(&ev0));
@@ -808,11 +871,11 @@ void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel
     for (int round = 0; round < max_rounds; round++) {
         if (cancelled(cancel)) fail(TS_CANCELLED, "cancelled");
         TS_HIP(hipMemsetAsync(t.status.p, 0, 2 * sizeof(int32_t), st));
-        const bool early = round >= early_from;
-        hipGraphExec_t ge = t.ctx->use_graphs ? round_graph<R>(t, T, count, early) : nullptr;
+        const int order = round_order(all_dirty, one_phase, round);
+        hipGraphExec_t ge = t.ctx->use_graphs ? round_graph<R>(t, T, count, order) : nullptr;
         TS_HIP(hipEventRecord(ev0, st));
         if (ge) TS_HIP(hipGraphLaunch(ge, st));
-        else launch_round_as<R>(t, T, st, hold, count, early);
+        else launch_round_as<R>(t, T, st, hold, count, order);
         TS_HIP(hipEventRecord(ev1, st));
         TS_HIP(hipMemcpyAsync(h_status, t.status.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
         TS_HIP(hipStreamSynchronize(st));
@@ -912,7 +975,7 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     t.wake_tol[0] = 0.0;
     t.wake_tol[1] = mixed ? mixed_wake_tol() : 0.0;
     if (mixed) {
-        run_rounds<float>(t, t.T32.p, count, cancel, h_status, false);
+        run_rounds<float>(t, t.T32.p, count, cancel, h_status, false, true);
         eik_mixed_widen<<<blocks_for(nodes_total * kLanes, 256), 256, 0, st>>>(t.T32.p, t.T64.p, nodes_total * kLanes);
         // the fixed seed times again, unrounded
         eik_fast_seed<double><<<sb, 256, 0, st>>>(t.T64.p, t.seedmask.p, t.dirty.p, df.p, dn.p, dv.p, (int)sf.size(),
@@ -921,7 +984,7 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
         eik_fill_u8<<<blocks_for(nodes_total, 256), 256, 0, st>>>(t.dirty.p, nodes_total, 1);
         TS_HIP(hipGetLastError());
     }
-    run_rounds<double>(t, t.T64.p, count, cancel, h_status, mixed);
+    run_rounds<double>(t, t.T64.p, count, cancel, h_status, mixed, mixed);
     dim3 ug(blocks_for(N, 64), groups);
     eik_fast_unpack<double><<<ug, 256, 0, st>>>(t.T64.p, N, t.field_of_slot_dev.p, t.tab32.p, t.tab64.p);
     TS_HIP(hipGetLastError());
