@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtomostar_hip.so")
+LIB_PATH = os.environ.get("TOMOSTAR_LIB") or os.path.join(HERE, "libtomostar_hip.so")  # TOMOSTAR_LIB: another build
 
 TS_OK, TS_DECLINED, TS_EINVAL, TS_CANCELLED, TS_EDEVICE = 0, 1, -1, -2, -3
 TS_CREATE_SKIP_SELFTEST = 1

@@ -43,6 +43,10 @@ struct LsqrScal {
 };
 
 constexpr int kSpmmBlocks = 2048;
+#ifndef TS_GATHER_BATCH
+#define TS_GATHER_BATCH 16
+#endif
+constexpr int kGatherBatch = TS_GATHER_BATCH;  // FP64 gathers a row's lanes issue together (FP32: half)
 
 // ------------------------------------------------------------------ matrix upload and transpose
 
@@ -128,6 +132,13 @@ __global__ __launch_bounds__(256) void mx_perm_rows(const int32_t *__restrict__ 
     }
 }
 
+// ci[q] = map[ci[q]]: column indices into the stored column order
+__global__ void mx_map_cols(int32_t *__restrict__ ci, int64_t nnz, const int32_t *__restrict__ map)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nnz; q += (int64_t)gridDim.x * blockDim.x)
+        ci[q] = map[ci[q]];
+}
+
 // dst[r] = src[perm[r]] over m rows
 __global__ void mx_permute_vec(const double *__restrict__ src, const int32_t *__restrict__ perm, int m,
                                double *__restrict__ dst)
@@ -167,10 +178,21 @@ static void reorder_rows(ts_matrix &A, dbuf<double> &va64, hipStream_t st)
     va64 = std::move(nva);
 }
 
-void matrix_finish(ts_matrix &A, dbuf<double> &&d_va64, bool keep_fp64)
+void matrix_finish(ts_matrix &A, dbuf<double> &&d_va64, bool keep_fp64, const int32_t *stored_col)
 {
     hipStream_t st = A.ctx->stream;
     const int64_t nnz = A.nnz;
+    if (stored_col && A.cols > 0) {
+        std::vector<int32_t> given(A.cols);
+        for (int c = 0; c < A.cols; c++) given[stored_col[c]] = c;
+        dbuf<int32_t> map(A.cols);
+        A.col_perm.alloc(A.cols);
+        TS_HIP(hipMemcpyAsync(map.p, stored_col, (size_t)A.cols * 4, hipMemcpyHostToDevice, st));
+        TS_HIP(hipMemcpyAsync(A.col_perm.p, given.data(), (size_t)A.cols * 4, hipMemcpyHostToDevice, st));
+        if (nnz > 0) mx_map_cols<<<grid_for(nnz), 256, 0, st>>>(A.ci.p, nnz, map.p);
+        TS_HIP(hipGetLastError());
+        TS_HIP(hipStreamSynchronize(st));
+    }
     if (nnz > 0 && A.rows > 1) reorder_rows(A, d_va64, st);
     A.trs.alloc((size_t)A.cols + 1);
     A.tci.alloc(std::max<int64_t>(1, nnz));
@@ -223,22 +245,67 @@ __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__rest
     constexpr int G = 256 / KP;
     const int j = threadIdx.x % KP, grp = threadIdx.x / KP;
     const double sx = sc[j], cy = sc[KP + j];
+    // Block b runs on XCD b mod 8. A step of the grid covers gridDim.x * G consecutive rows, cut into
+    // eight contiguous chunks, one per XCD: neighbouring rows of the stored order (rays between the
+    // same regions, cells of one region) gather through one L2, and the eight chunks of a step are
+    // neighbours in the shared last-level cache.
+    const int chunk = (gridDim.x / 8) * G, lr = (blockIdx.x / 8) * G + grp;
     double sq = 0;
-    for (int r = blockIdx.x * G + grp; r < rows; r += gridDim.x * G) {
+    for (int base = (blockIdx.x % 8) * chunk; base < rows; base += 8 * chunk) {
+        const int r = base + lr;
+        if (r >= rows) break;
         const int a = rs[r], b = rs[r + 1];
         double acc0 = 0, acc1 = 0;
-        int q = a;
-        for (; q + 4 <= b; q += 4) {
-            const int c0 = ci[q], c1 = ci[q + 1], c2 = ci[q + 2], c3 = ci[q + 3];
-            const float v0 = va[q], v1 = va[q + 1], v2 = va[q + 2], v3 = va[q + 3];
-            const T x0 = X[(size_t)c0 * KP + j], x1 = X[(size_t)c1 * KP + j];
-            const T x2 = X[(size_t)c2 * KP + j], x3 = X[(size_t)c3 * KP + j];
-            acc0 += (double)v0 * (double)x0;
-            acc1 += (double)v1 * (double)x1;
-            acc0 += (double)v2 * (double)x2;
-            acc1 += (double)v3 * (double)x3;
+        if constexpr (KP >= 8) {
+            // The KP lanes of the row load KP column indices and values at once (one coalesced load
+            // each) and pass them round with shuffles; the gathers of kGatherBatch entries are issued
+            // together, so a wave keeps many gathers in flight instead of waiting on each index.
+            constexpr int kBatch = sizeof(T) == 8 ? kGatherBatch : kGatherBatch / 2;
+            constexpr int GB = KP < kBatch ? KP : kBatch;
+            int cn = 0;
+            float vn = 0.0f;
+            if (j < b - a) { cn = ci[a + j]; vn = va[a + j]; }
+            for (int q0 = a; q0 < b; q0 += KP) {
+                const int cnt = min(KP, b - q0);
+                const int c = cn;
+                const float v = vn;
+                // the next chunk's indices and values load while this chunk gathers
+                if (q0 + KP + j < b) { cn = ci[q0 + KP + j]; vn = va[q0 + KP + j]; }
+                int t = 0;
+                for (; t + GB <= cnt; t += GB) {
+                    T x[GB];
+                    float w[GB];
+#pragma unroll
+                    for (int u = 0; u < GB; u++) {
+                        const int col = __shfl(c, t + u, KP);
+                        w[u] = __shfl(v, t + u, KP);
+                        x[u] = X[(size_t)col * KP + j];
+                    }
+#pragma unroll
+                    for (int u = 0; u < GB; u += 2) {
+                        acc0 += (double)w[u] * (double)x[u];
+                        acc1 += (double)w[u + 1] * (double)x[u + 1];
+                    }
+                }
+                for (; t < cnt; t++) {
+                    const int col = __shfl(c, t, KP);
+                    acc0 += (double)__shfl(v, t, KP) * (double)X[(size_t)col * KP + j];
+                }
+            }
+        } else {
+            int q = a;
+            for (; q + 4 <= b; q += 4) {
+                const int c0 = ci[q], c1 = ci[q + 1], c2 = ci[q + 2], c3 = ci[q + 3];
+                const float v0 = va[q], v1 = va[q + 1], v2 = va[q + 2], v3 = va[q + 3];
+                const T x0 = X[(size_t)c0 * KP + j], x1 = X[(size_t)c1 * KP + j];
+                const T x2 = X[(size_t)c2 * KP + j], x3 = X[(size_t)c3 * KP + j];
+                acc0 += (double)v0 * (double)x0;
+                acc1 += (double)v1 * (double)x1;
+                acc0 += (double)v2 * (double)x2;
+                acc1 += (double)v3 * (double)x3;
+            }
+            for (; q < b; q++) acc0 += (double)va[q] * (double)X[(size_t)ci[q] * KP + j];
         }
-        for (; q < b; q++) acc0 += (double)va[q] * (double)X[(size_t)ci[q] * KP + j];
         const size_t at = (size_t)r * KP + j;
         const double old = cy != 0.0 ? cy * (double)Y[at] : 0.0;
         const T y = (T)(sx * (acc0 + acc1) - old);
@@ -431,13 +498,21 @@ __global__ void lsqr_load_b(int m, int k, int kp, const double *b, const int32_t
     U[q] = j < k ? (T)b[(size_t)j * m + (perm ? perm[r] : (int)r)] : (T)0;
 }
 
-__global__ void lsqr_unpack_x(int n, int k, int kp, const double *Xs, double *x)
+// x[j][given column] from the stored [column][kp] solution (col_perm null: the same order)
+__global__ void lsqr_unpack_x(int n, int k, int kp, const double *Xs, const int32_t *col_perm, double *x)
 {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (size_t)n * k) return;
     const int j = (int)(q / n);
     const size_t r = q - (size_t)j * n;
-    x[q] = Xs[r * kp + j];
+    x[(size_t)j * n + (col_perm ? col_perm[r] : (int)r)] = Xs[r * kp + j];
+}
+
+// dst[perm[c]] = src[c] over n columns
+__global__ void mx_unpermute_vec(const double *__restrict__ src, const int32_t *__restrict__ perm, int n,
+                                 double *__restrict__ dst)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) dst[perm[c]] = src[c];
 }
 
 // ------------------------------------------------------------------------------- plain kernels
@@ -611,7 +686,12 @@ void lsqr_plain(ts_matrix &A, int k, const double *b, const double *damp, int ma
                 }
             }
         }
-        TS_HIP(hipMemcpyAsync(o.x + (size_t)j * n, x.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+        if (A.col_perm.p) {  // the solution in the given column order
+            mx_unpermute_vec<<<grid_for(n), 256, 0, st>>>(x.p, A.col_perm.p, n, tn.p);
+            TS_HIP(hipMemcpyAsync(o.x + (size_t)j * n, tn.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+        } else {
+            TS_HIP(hipMemcpyAsync(o.x + (size_t)j * n, x.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+        }
         TS_HIP(hipStreamSynchronize(st));
         o.iterations[j] = it_done;
         o.stop[j] = stop;
@@ -742,7 +822,7 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     }
     TS_HIP(hipEventRecord(ev1, st));
     dbuf<double> xo((size_t)n * k);
-    lsqr_unpack_x<<<nblk((size_t)n * k, 256), 256, 0, st>>>(n, k, KP, F.X.p, xo.p);
+    lsqr_unpack_x<<<nblk((size_t)n * k, 256), 256, 0, st>>>(n, k, KP, F.X.p, A.col_perm.p, xo.p);
     TS_HIP(hipMemcpyAsync(o.x, xo.p, (size_t)n * k * 8, hipMemcpyDeviceToHost, st));
     TS_HIP(hipMemcpyAsync(cur.data(), F.S.p, KP * sizeof(LsqrScal), hipMemcpyDeviceToHost, st));
     if (o.hist && F.hist.p) TS_HIP(hipMemcpyAsync(o.hist, F.hist.p, (size_t)k * max_it * 8, hipMemcpyDeviceToHost, st));

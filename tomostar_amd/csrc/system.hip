@@ -356,6 +356,34 @@ void axis_weights(const Grid &g, double vertical, bool in_nodes, double w[3])
     w[2] = vertical * std::pow(h / hz, 2);
 }
 
+// Stored column order of an assembled system (the stored position of each column): the nodes of each
+// model block in tiles of 8 x 8 x 8 (TOMOSTAR_COL_TILE="XxYxZ" picks another shape, "0" keeps the
+// given order), tiles in grid order, nodes in grid order within a tile; other columns stay where they
+// are. Neighbouring cells then sit in neighbouring columns, so the rows of A^T that a stretch of the
+// grid's rays cross are products of neighbouring blocks and gather the same rows of the rays' vectors.
+std::vector<int32_t> stored_columns(const Grid &g, int cols, std::initializer_list<int> blocks_at)
+{
+    int tx = 8, ty = 8, tz = 8;
+    if (const char *e = getenv("TOMOSTAR_COL_TILE")) {
+        if (std::sscanf(e, "%dx%dx%d", &tx, &ty, &tz) != 3 || tx < 1 || ty < 1 || tz < 1) return {};
+    }
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    std::vector<int32_t> pos(cols);
+    for (int c = 0; c < cols; c++) pos[c] = c;
+    for (const int off : blocks_at) {
+        if (off < 0 || off + n > cols) continue;
+        int32_t at = off;
+        for (int k0 = 0; k0 < g.nz; k0 += tz)
+            for (int j0 = 0; j0 < g.ny; j0 += ty)
+                for (int i0 = 0; i0 < g.nx; i0 += tx)
+                    for (int k = k0; k < std::min(g.nz, k0 + tz); k++)
+                        for (int j = j0; j < std::min(g.ny, j0 + ty); j++)
+                            for (int i = i0; i < std::min(g.nx, i0 + tx); i++)
+                                pos[off + ((int64_t)k * g.ny + j) * g.nx + i] = at++;
+    }
+    return pos;
+}
+
 template <typename T>
 void upload(dbuf<T> &d, const T *h, size_t n, hipStream_t st)
 {
@@ -546,7 +574,10 @@ int ts_system_build(ts_context *ctx, const ts_grid *grid, const ts_system_opts *
         M->rhs.resize(m);
         if (m > 0) TS_HIP(hipMemcpyAsync(M->rhs.data(), rhs.p, m * 8, hipMemcpyDeviceToHost, st));
         TS_HIP(hipStreamSynchronize(st));
-        matrix_finish(*M, std::move(va), keep_fp64 != 0);
+        {
+            const std::vector<int32_t> order = stored_columns(g, (int)col, {(int)offP, (int)offS});
+            matrix_finish(*M, std::move(va), keep_fp64 != 0, order.empty() ? nullptr : order.data());
+        }
         M->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (layout) {
             const int64_t L[8] = {offP, offS, offH, offCP, offCS, col, m, data_rows};
@@ -595,6 +626,11 @@ int ts_matrix_read(ts_matrix *a, int32_t *row_start, int32_t *column, double *va
         }
         if (a->row_perm.p) TS_HIP(hipMemcpy(perm.data(), a->row_perm.p, (size_t)m * 4, hipMemcpyDeviceToHost));
         else for (int r = 0; r < m; r++) perm[r] = r;
+        if (a->col_perm.p && nnz > 0) {  // stored column indices back to the given ones
+            std::vector<int32_t> cp(a->cols);
+            TS_HIP(hipMemcpy(cp.data(), a->col_perm.p, (size_t)a->cols * 4, hipMemcpyDeviceToHost));
+            for (int64_t q = 0; q < nnz; q++) ci[q] = cp[ci[q]];
+        }
         std::vector<int32_t> where(std::max(1, m));  // stored position of each given row
         for (int r = 0; r < m; r++) where[perm[r]] = r;
         int64_t at = 0;
@@ -636,6 +672,23 @@ int ts_matrix_read_transpose(ts_matrix *a, int32_t *row_start, int32_t *column, 
                 std::sort(e.begin(), e.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
                 for (size_t i = 0; i < e.size(); i++) { tci[trs[c] + i] = e[i].first; tva[trs[c] + i] = e[i].second; }
             }
+        }
+        if (a->col_perm.p && n > 0) {
+            // transposed rows back in the given column order
+            std::vector<int32_t> cp(n), orow(n + 1, 0), oci(std::max<int64_t>(1, nnz));
+            std::vector<float> ova(std::max<int64_t>(1, nnz));
+            TS_HIP(hipMemcpy(cp.data(), a->col_perm.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+            std::vector<int32_t> stored(n);
+            for (int c = 0; c < n; c++) stored[cp[c]] = c;
+            for (int g = 0; g < n; g++) {
+                const int c = stored[g];
+                orow[g + 1] = orow[g] + (trs[c + 1] - trs[c]);
+                std::copy(tci.begin() + trs[c], tci.begin() + trs[c + 1], oci.begin() + orow[g]);
+                std::copy(tva.begin() + trs[c], tva.begin() + trs[c + 1], ova.begin() + orow[g]);
+            }
+            trs.swap(orow);
+            tci.swap(oci);
+            tva.swap(ova);
         }
         if (row_start) std::copy(trs.begin(), trs.end(), row_start);
         if (column && nnz > 0) std::copy(tci.begin(), tci.begin() + nnz, column);
