@@ -192,6 +192,11 @@ int ts_matrix_read(ts_matrix *a, int32_t *row_start, int32_t *column, double *va
 /* The stored transpose (CsrMatrix.Transpose, SparseMatrix.cs:79-97): row_start[cols+1], column[nnz]
  * (each transposed row's rows ascending) and its FP32 values. */
 int ts_matrix_read_transpose(ts_matrix *a, int32_t *row_start, int32_t *column, float *values);
+/* y = A x (transpose = 0) or y = A^T x (transpose != 0) for k vectors on the device, FP64 sums over the
+ * FP64 values when kept, else the FP32 ones (CsrMatrix.MultiplyInto, Numerics/SparseMatrix.cs:46-73;
+ * ResolutionDiagonal.Estimate's A z and TradeOff's G x, Tomography/ResolutionDiagonal.cs:48,
+ * TravelTimeTomography.cs:599). x[k][cols or rows], y[k][rows or cols], in the given orders. */
+int ts_matrix_multiply(ts_matrix *a, int k, const double *x, double *y, int transpose, char *reason, int reason_len);
 void ts_matrix_release(ts_matrix *a);
 
 /* ---------------------------------------------------------------- the linear system

@@ -281,6 +281,32 @@ def lsqr(m, n, rs, ci, va, b, damp=0.0, max_it=500, atol=1e-8, btol=1e-8, conlim
     return x, stop, {"iterations": it, "bnorm": stats[1], "rnorm": stats[2], "acond": stats[3]}, hist[:it].copy()
 
 
+def resolution_diagonal(m, n, rs, ci, va, data_rows, signs, lsqr_iterations, tolerance, threads=0):
+    """ResolutionDiagonal.Estimate (Tomography/ResolutionDiagonal.cs:31-73) with the probes given:
+    signs[probes][n] of +-1. Per probe: az = A z (:48), rhs = az on the first data_rows rows and 0 on
+    the rest (:50-51), Lsqr.Solve(A, rhs, damp 0, lsqr_iterations, tolerance, tolerance, 1e8) (:52),
+    e = z * x summed and squared (:54-59); then the mean, and the standard error
+    sqrt(max(0, (sum2 - p mean^2) / (p - 1)) / p) (:61-69). Returns (diagonal, stderr, mean iterations)."""
+    signs = np.asarray(signs, np.float64)
+    probes = signs.shape[0]
+    tr = csr_transpose(m, n, rs, ci, va)
+    s1 = np.zeros(n)
+    s2 = np.zeros(n)
+    its = 0
+    for z in signs:
+        az = spmv(rs, ci, va, z, threads)
+        rhs = np.zeros(m)
+        rhs[:data_rows] = az[:data_rows]
+        x, _, st, _ = lsqr(m, n, rs, ci, va, rhs, 0.0, lsqr_iterations, tolerance, tolerance, 1e8, threads, tr)
+        its += st["iterations"]
+        e = z * x
+        s1 += e
+        s2 += e * e
+    mean = s1 / probes
+    var = np.maximum(0.0, (s2 - probes * mean * mean) / (probes - 1))
+    return mean, np.sqrt(var / probes), its // probes
+
+
 # ---------------------------------------------------------------- tables and rays
 def table_time(g: GridDef, table, lon, lat, depth) -> float:
     return lib().or_table_time(C.byref(g), np.ascontiguousarray(table, np.float32), lon, lat, depth)

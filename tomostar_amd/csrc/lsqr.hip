@@ -139,6 +139,18 @@ __global__ void mx_map_cols(int32_t *__restrict__ ci, int64_t nnz, const int32_t
         ci[q] = map[ci[q]];
 }
 
+// y[r] = sum_q va[q] x[ci[q]] in FP64 (CsrMatrix.MultiplyInto, SparseMatrix.cs:46-73), one row per thread
+template <typename V>
+__global__ void mx_spmv(int rows, const int32_t *__restrict__ rs, const int32_t *__restrict__ ci,
+                        const V *__restrict__ va, const double *__restrict__ x, double *__restrict__ y)
+{
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+        double acc = 0;
+        for (int q = rs[r]; q < rs[r + 1]; q++) acc += (double)va[q] * x[ci[q]];
+        y[r] = acc;
+    }
+}
+
 // dst[r] = src[perm[r]] over m rows
 __global__ void mx_permute_vec(const double *__restrict__ src, const int32_t *__restrict__ perm, int m,
                                double *__restrict__ dst)
@@ -1063,6 +1075,43 @@ int ts_matrix_create(ts_context *ctx, int rows, int cols, const int32_t *row_sta
         matrix_finish(*A, std::move(va), keep_fp64 != 0);
         A->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         *out = A.release();
+        return TS_OK;
+    });
+}
+
+int ts_matrix_multiply(ts_matrix *a, int k, const double *x, double *y, int transpose, char *reason, int reason_len)
+{
+    if (!a || k < 0 || (k > 0 && (!x || !y))) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    return guarded(reason, reason_len, [&]() -> int {
+        TS_HIP(hipSetDevice(a->ctx->device));
+        std::lock_guard<std::mutex> lk(a->gate);
+        hipStream_t st = a->ctx->stream;
+        const int m = a->rows, n = a->cols;
+        const int nin = transpose ? m : n, nout = transpose ? n : m;
+        // permutation of the input into the stored order and of the output back to the given one
+        const int32_t *pin = transpose ? a->row_perm.p : a->col_perm.p;
+        const int32_t *pout = transpose ? a->col_perm.p : a->row_perm.p;
+        dbuf<double> hin(std::max(1, nin)), din(std::max(1, nin)), dout(std::max(1, nout)), hout(std::max(1, nout));
+        for (int j = 0; j < k; j++) {
+            TS_HIP(hipMemcpyAsync(hin.p, x + (size_t)j * nin, (size_t)nin * 8, hipMemcpyHostToDevice, st));
+            if (pin) mx_permute_vec<<<grid_for(nin), 256, 0, st>>>(hin.p, pin, nin, din.p);
+            const double *xin = pin ? din.p : hin.p;
+            double *yout = pout ? dout.p : hout.p;
+            if (transpose) {
+                if (a->tva64.p) mx_spmv<double><<<grid_for(n), 256, 0, st>>>(n, a->trs.p, a->tci.p, a->tva64.p, xin, yout);
+                else mx_spmv<float><<<grid_for(n), 256, 0, st>>>(n, a->trs.p, a->tci.p, a->tva32.p, xin, yout);
+            } else {
+                if (a->va64.p) mx_spmv<double><<<grid_for(m), 256, 0, st>>>(m, a->rs.p, a->ci.p, a->va64.p, xin, yout);
+                else mx_spmv<float><<<grid_for(m), 256, 0, st>>>(m, a->rs.p, a->ci.p, a->va32.p, xin, yout);
+            }
+            if (pout) mx_unpermute_vec<<<grid_for(nout), 256, 0, st>>>(dout.p, pout, nout, hout.p);
+            TS_HIP(hipGetLastError());
+            TS_HIP(hipMemcpyAsync(y + (size_t)j * nout, hout.p, (size_t)nout * 8, hipMemcpyDeviceToHost, st));
+            TS_HIP(hipStreamSynchronize(st));
+        }
         return TS_OK;
     });
 }

@@ -54,6 +54,8 @@ def _declare():
     L.ts_matrix_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(d)]
     L.ts_matrix_release.restype = None
     L.ts_matrix_release.argtypes = [vp]
+    L.ts_matrix_multiply.restype = i
+    L.ts_matrix_multiply.argtypes = [vp, i, vp, vp, i, C.c_char_p, i]
     L.ts_lsqr_solve.restype = i
     L.ts_lsqr_solve.argtypes = [vp, i, vp, vp, i, d, d, d, i, vp, vp, vp, vp, vp, vp, C.c_char_p, i]
     L.ts_hip_selftest_lsqr_fast.restype = i
@@ -92,6 +94,10 @@ class Matrix:
     def handle(self):
         return self._h
 
+    def multiply(self, x: np.ndarray, transpose: bool = False) -> np.ndarray:
+        """A x (or A^T x) on the device for x[cols] or x[k, cols] (CsrMatrix.MultiplyInto)."""
+        return multiply(self, x, transpose)
+
     def last_seconds(self) -> float:
         n, s = C.c_int64(), C.c_double()
         check(_declare().ts_matrix_stats(self._h, C.byref(n), C.byref(s)))
@@ -107,6 +113,24 @@ class Matrix:
             self.release()
         except Exception:
             pass
+
+
+def multiply(a: Matrix, x: np.ndarray, transpose: bool = False) -> np.ndarray:
+    """y = A x (transpose: A^T x) on the device in FP64 sums, x[n] or x[k, n] in the given orders
+    (CsrMatrix.MultiplyInto, Numerics/SparseMatrix.cs:46-73)."""
+    L = _declare()
+    x = np.asarray(x, np.float64)
+    single = x.ndim == 1
+    x2 = np.ascontiguousarray(np.atleast_2d(x))
+    nin, nout = (a.rows, a.cols) if transpose else (a.cols, a.rows)
+    if x2.shape[1] != nin:
+        raise ValueError(f"vector length {x2.shape[1]} differs from {nin}")
+    y = np.zeros((x2.shape[0], nout))
+    r = reason_buffer()
+    rc = L.ts_matrix_multiply(a.handle, x2.shape[0], ptr(x2), ptr(y), 1 if transpose else 0, r, len(r))
+    if rc != 0:
+        raise _lib.HipError(rc, r.value.decode(errors="replace"))
+    return y[0] if single else y
 
 
 def solve_many(a: Matrix, b: np.ndarray, damp: Sequence[float], max_iterations: int = 500, atol: float = 1e-8,
