@@ -196,10 +196,65 @@ def iteration_leg(dev, cpu: bool):
            "rows": r.rows, "columns": r.columns, "nnz": r.nnz, "data_rows": r.data_rows,
            "gpu_seconds": {k: round(v, 4) for k, v in r.stage_seconds.items()}}
     out["gpu_seconds"]["total"] = round(sum(r.stage_seconds.values()), 4)
+    out["consumers"] = consumers_leg(dev, wl, t_obs, r)
     if cpu:
         out["cpu_seconds"] = cpu_baseline_iteration(wl, r)
+        out["consumers"]["cpu_seconds"] = cpu_baseline_consumers(r, out["consumers"])
     log(f"iteration leg: {out}")
     return out
+
+
+def consumers_leg(dev, wl, t_obs, r):
+    """The multi-RHS consumers of the iteration's system: the resolution diagonal (32 probes, 100 LSQR
+    iterations each, one batched solve; the plain FP64 form one probe at a time, timed on 2 probes)
+    and a trade-off curve of 4 (damping, smoothing) pairs (assembly + 100 LSQR iterations per pair)."""
+    from tomostar_amd import lsqr as Q
+    from tomostar_amd import resolution as R
+    from tomostar_amd import tradeoff as T
+
+    probes, its = 32, 100
+    R.estimate(r.system.matrix, r.data_rows, 2, 5, 0.0, seed=1)  # warm-up
+    t0 = time.perf_counter()
+    res = R.estimate(r.system.matrix, r.data_rows, probes, its, 0.0, seed=1)
+    t_fast = time.perf_counter() - t0
+    z = R.probe_signs(r.columns, 2, seed=1)
+    rhs = Q.multiply(r.system.matrix, z)
+    rhs[:, r.data_rows:] = 0.0
+    t0 = time.perf_counter()
+    for j in range(2):
+        Q.solve_many(r.system.matrix, rhs[j][None, :], [0.0], its, 0.0, 0.0, 1e8, mode="plain_fp64")
+    t_plain = (time.perf_counter() - t0) / 2
+    pairs = [(10.0, 5.0), (3.0, 5.0), (1.0, 5.0), (0.3, 5.0)]
+    t0 = time.perf_counter()
+    pts = T.trade_off(dev, wl, t_obs, pairs, lsqr_iterations=its, lsqr_tolerance=0.0)
+    t_trade = time.perf_counter() - t0
+    return {"resolution_diagonal": {"probes": probes, "lsqr_iterations": its, "gpu_seconds": round(t_fast, 4),
+                                    "mean_diagonal": round(float(res.diagonal.mean()), 6),
+                                    "plain_fp64_gpu_seconds_per_probe": round(t_plain, 4),
+                                    "speedup_vs_plain_fp64_gpu": round(t_plain * probes / t_fast, 2)},
+            "trade_off": {"pairs": pairs, "lsqr_iterations": its, "gpu_seconds": round(t_trade, 4),
+                          "residual_norms": [round(p.residual_norm, 4) for p in pts],
+                          "model_norms": [round(p.model_norm, 4) for p in pts]}}
+
+
+def cpu_baseline_consumers(r, cons):
+    """The oracle's resolution diagonal on 2 of the probes (scaled to 32) and its LSQR per trade-off
+    pair (the assembly left out), on the host cores."""
+    from oracle import oracle as O
+    from tomostar_amd import resolution as R
+
+    cores = min(os.cpu_count() or 1, 16)
+    rs, ci, va = r.system.read()
+    z = R.probe_signs(r.columns, 2, seed=1)
+    t0 = time.perf_counter()
+    O.resolution_diagonal(r.rows, r.columns, rs, ci, va, r.data_rows, z, 100, 0.0, threads=cores)
+    t_probe = (time.perf_counter() - t0) / 2
+    t0 = time.perf_counter()
+    O.lsqr(r.rows, r.columns, rs, ci, va, r.rhs, damp=0.0, max_it=100, atol=0, btol=0, conlim=1e8, threads=cores)
+    t_pair = time.perf_counter() - t0
+    probes = cons["resolution_diagonal"]["probes"]
+    return {"resolution_diagonal": round(t_probe * probes, 3), "trade_off_lsqr": round(t_pair * len(cons["trade_off"]["pairs"]), 3),
+            "cores": cores, "sample": "2 probes scaled to 32; one LSQR per pair"}
 
 
 def cpu_baseline_iteration(wl, r):

@@ -362,22 +362,39 @@ __global__ __launch_bounds__(256) void lsqr_xw(int n, const T *__restrict__ V, T
     if (grp == 0) partial[(size_t)blockIdx.x * KP + j] = red[j];
 }
 
-__device__ inline double sum_partials(const double *p, int blocks, int kp, int j)
+// The per-RHS sums of the product kernels' per-block partials, by the whole workgroup (kFinThreads
+// threads, kp of them per stride): thread (part, j) sums blocks part, part + P, ... in order, then a
+// fixed tree over the parts; every thread returns the sum of its j. The order is fixed: the same bits
+// every run. Call from all threads (it synchronises).
+constexpr int kFinThreads = 1024;
+
+__device__ inline double sum_partials(const double *p, int blocks, int kp, double *red)
 {
+    const int t = threadIdx.x, j = t % kp, part = t / kp, parts = blockDim.x / kp;
     double s = 0;
-    for (int b = 0; b < blocks; b++) s += p[(size_t)b * kp + j];
-    return s;
+    for (int b = part; b < blocks; b += parts) s += p[(size_t)b * kp + j];
+    __syncthreads();  // red may hold an earlier call's values
+    red[t] = s;
+    __syncthreads();
+    for (int h = parts / 2; h > 0; h >>= 1) {
+        if (part < h) red[t] += red[t + h * kp];
+        __syncthreads();
+    }
+    return red[j];
 }
 
 // After A v: beta, the u scale, ||A||^2 and the coefficients of the A^T product (Lsqr.cs:79-87).
-__global__ void lsqr_fin_u(int k, int kp, LsqrScal *S, const double *partial, int blocks, double *sc_t)
+__global__ __launch_bounds__(kFinThreads) void lsqr_fin_u(int k, int kp, LsqrScal *S, const double *partial, int blocks,
+                                                         double *sc_t)
 {
+    __shared__ double red[kFinThreads];
+    const double sum = sum_partials(partial, blocks, kp, red);
     const int j = threadIdx.x;
     if (j >= kp) return;
     if (j >= k) { sc_t[j] = 0; sc_t[kp + j] = -1; return; }
     LsqrScal s = S[j];
     if (s.done) { sc_t[j] = 0; sc_t[kp + j] = -1; return; }
-    const double beta = sqrt(sum_partials(partial, blocks, kp, j));
+    const double beta = sqrt(sum);
     s.beta = beta;
     if (beta > 0) {
         s.su = 1.0 / beta;
@@ -395,9 +412,13 @@ __global__ void lsqr_fin_u(int k, int kp, LsqrScal *S, const double *partial, in
 
 // After A^T u: alpha, the v scale, the plane rotations, the norm estimates and the stopping rules of
 // one iteration (Lsqr.cs:88-146), and the coefficients of the next A v product and of the x/w update.
-__global__ void lsqr_fin_v(int k, int kp, LsqrScal *S, const double *partial_v, const double *partial_w,
-                           int blocks_v, int blocks_w, double *sc_a, double *hist, int max_it)
+__global__ __launch_bounds__(kFinThreads) void lsqr_fin_v(int k, int kp, LsqrScal *S, const double *partial_v,
+                                                         const double *partial_w, int blocks_v, int blocks_w,
+                                                         double *sc_a, double *hist, int max_it)
 {
+    __shared__ double red[kFinThreads];
+    const double sum_w = sum_partials(partial_w, blocks_w, kp, red);
+    const double sum_v = sum_partials(partial_v, blocks_v, kp, red);
     const int j = threadIdx.x;
     if (j >= kp) return;
     if (j >= k) { sc_a[j] = 0; sc_a[kp + j] = -1; return; }
@@ -411,9 +432,9 @@ __global__ void lsqr_fin_v(int k, int kp, LsqrScal *S, const double *partial_v, 
         sc_a[kp + j] = -1;
         return;
     }
-    const double wnorm2 = sum_partials(partial_w, blocks_w, kp, j);
+    const double wnorm2 = sum_w;
     if (!s.skip) {
-        s.alpha = sqrt(sum_partials(partial_v, blocks_v, kp, j));
+        s.alpha = sqrt(sum_v);
         if (s.alpha > 0) s.sv = 1.0 / s.alpha;
     }
     const double alpha = s.alpha, beta = s.beta, damp = s.damp;
@@ -473,14 +494,17 @@ __global__ void lsqr_fin_v(int k, int kp, LsqrScal *S, const double *partial_v, 
 
 // Start of the solve (Lsqr.cs:51-66) once U = b and beta = ||b|| are set and V = A^T U: alpha, the
 // first w = v and the scalars.
-__global__ void lsqr_fin_init(int k, int kp, LsqrScal *S, const double *partial_v, int blocks, double *sc_a)
+__global__ __launch_bounds__(kFinThreads) void lsqr_fin_init(int k, int kp, LsqrScal *S, const double *partial_v,
+                                                            int blocks, double *sc_a)
 {
+    __shared__ double red[kFinThreads];
+    const double sum = sum_partials(partial_v, blocks, kp, red);
     const int j = threadIdx.x;
     if (j >= kp) return;
     if (j >= k) { sc_a[j] = 0; sc_a[kp + j] = -1; return; }
     LsqrScal s = S[j];
     if (!s.done) {
-        s.alpha = sqrt(sum_partials(partial_v, blocks, kp, j));
+        s.alpha = sqrt(sum);
         if (s.alpha == 0) {
             s.done = 1;
             s.stop = 10;
@@ -726,9 +750,9 @@ template <int KP, typename T>
 void fast_iteration(ts_matrix &A, FastLsqr<T> &F, hipStream_t st, bool with_hist)
 {
     lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.rows, A.rs.p, A.ci.p, A.va32.p, F.V.p, F.U.p, F.sc_a.p, F.pu.p);
-    lsqr_fin_u<<<1, 64, 0, st>>>(F.k, KP, F.S.p, F.pu.p, kSpmmBlocks, F.sc_t.p);
+    lsqr_fin_u<<<1, kFinThreads, 0, st>>>(F.k, KP, F.S.p, F.pu.p, kSpmmBlocks, F.sc_t.p);
     lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.cols, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
-    lsqr_fin_v<<<1, 64, 0, st>>>(F.k, KP, F.S.p, F.pv.p, F.pw.p, kSpmmBlocks, kSpmmBlocks, F.sc_a.p,
+    lsqr_fin_v<<<1, kFinThreads, 0, st>>>(F.k, KP, F.S.p, F.pv.p, F.pw.p, kSpmmBlocks, kSpmmBlocks, F.sc_a.p,
                                  with_hist ? F.hist.p : nullptr, F.max_it);
     lsqr_xw<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.cols, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
 }
@@ -792,7 +816,7 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     TS_HIP(hipMemcpyAsync(F.sc_t.p, sc.data(), 2 * KP * 8, hipMemcpyHostToDevice, st));
     // v = A^T u, alpha, w = v
     lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(n, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
-    lsqr_fin_init<<<1, 64, 0, st>>>(k, KP, F.S.p, F.pv.p, kSpmmBlocks, F.sc_a.p);
+    lsqr_fin_init<<<1, kFinThreads, 0, st>>>(k, KP, F.S.p, F.pv.p, kSpmmBlocks, F.sc_a.p);
     lsqr_xw<KP, T><<<kSpmmBlocks, 256, 0, st>>>(n, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
     TS_HIP(hipGetLastError());
     TS_HIP(hipStreamSynchronize(st));
