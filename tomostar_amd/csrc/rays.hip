@@ -284,33 +284,53 @@ __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
         }
     }
     __syncthreads();
-    // the row: occupied slots, ascending by node (bitonic network over kHashSlots keys in LDS)
-    for (int size = 2; size <= kHashSlots; size <<= 1)
+    // the row: occupied slots, ascending by node. The occupied slots are moved to the front of the
+    // table in slot order, 64 slots at a time (all lanes read their slot, then write it at or below
+    // the slots read so far), and only the next power of two of their count is sorted, with a
+    // bitonic network.
+    int count = 0;
+    for (int base = 0; base < kHashSlots; base += 64) {
+        const int32_t k = hkey[base + lane];
+        const double v = hval[base + lane];
+        const uint64_t occ = __ballot(k != -1);
+        __syncthreads();
+        if (k != -1) {
+            const int at = count + __popcll(occ & ((1ull << lane) - 1));
+            hkey[at] = k;
+            hval[at] = v;
+        }
+        count += __popcll(occ);
+        __syncthreads();
+    }
+    if (count > A.row_cap || overflow) {
+        if (lane == 0) A.row_count[slot] = -3;
+        return;
+    }
+    int span = 64;
+    while (span < count) span <<= 1;
+    for (int i = count + lane; i < span; i += 64) hkey[i] = -1;  // pads sort last
+    __syncthreads();
+    int32_t *key = hkey;
+    double *val = hval;
+    for (int size = 2; size <= span; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = lane; i < kHashSlots; i += 64) {
+            for (int i = lane; i < span; i += 64) {
                 const int jx = i ^ stride;
                 if (jx > i) {
                     const bool asc = (i & size) == 0;
-                    // empty slots (-1) sort last
-                    const uint32_t ki = (uint32_t)hkey[i], kj = (uint32_t)hkey[jx];
+                    // empty slots and pads (-1) sort last
+                    const uint32_t ki = (uint32_t)key[i], kj = (uint32_t)key[jx];
                     if ((ki > kj) == asc) {
-                        hkey[i] = (int32_t)kj; hkey[jx] = (int32_t)ki;
-                        const double t = hval[i]; hval[i] = hval[jx]; hval[jx] = t;
+                        key[i] = (int32_t)kj; key[jx] = (int32_t)ki;
+                        const double t = val[i]; val[i] = val[jx]; val[jx] = t;
                     }
                 }
             }
             __syncthreads();
         }
-    int count = 0;
-    for (int i = lane; i < kHashSlots; i += 64) count += hkey[i] != -1;
-    for (int off = 32; off > 0; off >>= 1) count += __shfl_xor(count, off);
-    if (count > A.row_cap || overflow) {
-        if (lane == 0) A.row_count[slot] = -3;
-        return;
-    }
     for (int i = lane; i < count; i += 64) {
-        A.row_nodes[(size_t)slot * A.row_cap + i] = hkey[i];
-        A.row_len[(size_t)slot * A.row_cap + i] = (float)hval[i];
+        A.row_nodes[(size_t)slot * A.row_cap + i] = key[i];
+        A.row_len[(size_t)slot * A.row_cap + i] = (float)val[i];
     }
     if (lane == 0) A.row_count[slot] = count;
 }
