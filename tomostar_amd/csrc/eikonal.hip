@@ -35,8 +35,14 @@ namespace ts {
 
 constexpr double kSettleTol = 1e-7;  // SphericalEikonal.cs:332 SettleTolerance
 constexpr int kLanes = 64;
+// The mixed form's first-order phase on FP32 times counts a node as moved when it dropped by more than
+// the settle tolerance and by more than this many FP32 ulps of its time: an FP32 time of tens of seconds
+// cannot move by less than an ulp (~2e-6 s), so without the ulp floor the phase runs an extra round on
+// rounding-level drops. The phase only seeds the FP64 second-order phase, which converges on its own:
+// measured on 400 fields of 201 x 201 x 101, 4 ulps end the phase one round earlier (5 instead of 6)
+// with the result's distance to the FP64 form unchanged.
 #ifndef TS_FP32_SETTLE_ULPS
-#define TS_FP32_SETTLE_ULPS 0
+#define TS_FP32_SETTLE_ULPS 4
 #endif
 
 inline int sweep_round_cap(const Grid &g)  // SphericalEikonal.cs:335
