@@ -82,6 +82,11 @@ def test_invalid_arguments_are_rejected():
     assert L.ts_eik_suggest_batch(None, 1000) == _lib.TS_EINVAL
     r = _lib.reason_buffer()
     assert L.ts_tables_create(None, None, 1, None, 1, None, None, None, None, None, r, len(r)) == _lib.TS_EINVAL
+    from tomostar_amd import lsqr as Q
+
+    Q._declare()
+    assert L.ts_matrix_multiply(None, 1, None, None, 0, r, len(r)) == _lib.TS_EINVAL
+    assert b"invalid argument" in r.value
 
 
 def test_system_build_marshalling_without_a_device():
