@@ -175,6 +175,21 @@ int ts_rays_rows(ts_rays *r, int32_t *counts, int64_t *row_start, int32_t *nodes
 int ts_rays_stats(ts_rays *r, int64_t *nnz, double *seconds);
 void ts_rays_release(ts_rays *r);
 
+/* ---------------------------------------------------------------- event location
+ * EventLocator.GridSearch (Location/EventLocator.cs:114-175, WeightedMedian :311-323) over the tables
+ * of a solved table set, for many events at once: per event the picks
+ * [pick_start[e], pick_start[e+1]) with their table (field index), observed time relative to the
+ * event's reference (s), weight and station correction. center[3e..3e+2] (i, j, k), when given, makes
+ * the search local (every node within 6 of it; LocatorSettings.GlobalSearch = false); otherwise a
+ * strided sub-grid of about coarse_candidates nodes, then every node within two strides of the best.
+ * k_fixed[e] >= 0 fixes the depth index (FixDepth). Out: the best node index, its origin time (the
+ * weighted median) and its misfit, equal to the reference's. At most 512 picks per event (more:
+ * declined). */
+int ts_locate_grid_search(ts_tableset *t, int nevents, const int32_t *pick_start, const int32_t *pick_field,
+                          const double *obs, const double *weight, const double *correction, int coarse_candidates,
+                          const int32_t *center, const int32_t *k_fixed, int32_t *best_node, double *best_t0,
+                          double *best_misfit, char *reason, int reason_len);
+
 /* ---------------------------------------------------------------- LSQR
  * A CSR matrix resident in HBM with its stored transpose (CsrMatrix and CsrMatrix.Transpose,
  * Numerics/SparseMatrix.cs:24-97; int32 indices as CsrMatrix's). row_start[rows+1], column[nnz],

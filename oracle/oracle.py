@@ -307,6 +307,64 @@ def resolution_diagonal(m, n, rs, ci, va, data_rows, signs, lsqr_iterations, tol
     return mean, np.sqrt(var / probes), its // probes
 
 
+def weighted_median(values, weights):
+    """EventLocator.WeightedMedian (Location/EventLocator.cs:311-323): the finite values in a stable sort,
+    their weights summed in that order, the first value whose running weight reaches half; 0 when none."""
+    v = np.asarray(values, np.float64)
+    w = np.asarray(weights, np.float64)
+    keep = ~np.isnan(v)
+    v, w = v[keep], w[keep]
+    if v.size == 0:
+        return 0.0
+    o = np.argsort(v, kind="stable")
+    acc = np.cumsum(w[o])  # sequential, in sorted order
+    half = acc[-1] / 2
+    hit = np.flatnonzero(acc >= half)
+    return float(v[o][hit[0]]) if hit.size else float(v[o][-1])
+
+
+def locate_grid_search(tables, nx, ny, nz, picks, coarse_candidates=8000, center=None, k_fixed=-1):
+    """EventLocator.GridSearch (EventLocator.cs:114-175) for one event: tables[field] float32 [N];
+    picks = (field, obs, weight, correction) arrays. Misfit per node: r = obs - t - correction (NaN where
+    the table is not finite), t0 = weighted median of r, m = sum in pick order of 10 w (NaN) or
+    w |r - t0|. A global search scans the stride grid (stride = max(1, ceil(cbrt(N / candidates))))
+    then every node within two strides of the best; a local search (center = (i, j, k)) every node
+    within 6; k_fixed >= 0 fixes the depth index. Returns (node, t0, misfit)."""
+    field, obs, w, corr = (np.asarray(a) for a in picks)
+    N = nx * ny * nz
+
+    def misfits(idx):
+        t = np.stack([tables[f][idx] for f in field], axis=1).astype(np.float64)  # [cand][pick]
+        r = np.where(np.isfinite(t), (obs[None, :] - t) - corr[None, :], np.nan)
+        t0 = np.array([weighted_median(row, w) for row in r])
+        terms = np.where(np.isnan(r), 10 * w[None, :], w[None, :] * np.abs(r - t0[:, None]))
+        return np.cumsum(terms, axis=1)[:, -1] if terms.shape[1] else np.zeros(len(idx)), t0
+
+    def search(i0, i1, j0, j1, k0, k1, stride):
+        i0, j0, k0 = max(0, i0), max(0, j0), max(0, k0)
+        i1, j1, k1 = min(nx - 1, i1), min(ny - 1, j1), min(nz - 1, k1)
+        if k_fixed >= 0:
+            k0 = k1 = k_fixed
+        kk, jj, ii = np.meshgrid(np.arange(k0, k1 + 1, stride), np.arange(j0, j1 + 1, stride),
+                                 np.arange(i0, i1 + 1, stride), indexing="ij")
+        idx = ((kk * ny + jj) * nx + ii).ravel()
+        if idx.size == 0:
+            return (k0 * ny + j0) * nx + i0
+        m, _ = misfits(idx)
+        return int(idx[int(np.argmin(m))])  # first minimum in k, j, i order
+
+    if center is not None:
+        ci, cj, ck = center
+        best = search(ci - 6, ci + 6, cj - 6, cj + 6, ck - 6, ck + 6, 1)
+    else:
+        s = max(1, int(np.ceil(np.cbrt(N / max(1, coarse_candidates)))))
+        best = search(0, nx - 1, 0, ny - 1, 0, nz - 1, s)
+        bi, bj, bk = best % nx, (best // nx) % ny, best // (nx * ny)
+        best = search(bi - 2 * s, bi + 2 * s, bj - 2 * s, bj + 2 * s, bk - 2 * s, bk + 2 * s, 1)
+    m, t0 = misfits(np.array([best]))
+    return best, float(t0[0]), float(m[0])
+
+
 # ---------------------------------------------------------------- tables and rays
 def table_time(g: GridDef, table, lon, lat, depth) -> float:
     return lib().or_table_time(C.byref(g), np.ascontiguousarray(table, np.float32), lon, lat, depth)
