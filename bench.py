@@ -204,6 +204,50 @@ def iteration_leg(dev, cpu: bool):
     return out
 
 
+def location_leg(wl, ts, args, cpu: bool):
+    """Event location's grid search over the 400 resident tables of the headline solve: events at random
+    nodes with 40 picks each (random station x phase tables, origin time, 0.05 s noise, quarter/half/
+    full weights, station corrections), global search with 8000 coarse candidates (stride 8 on this
+    grid, then every node within 16). The oracle's restatement timed on a sample of the events."""
+    from tomostar_amd import locate as Lc
+
+    g = wl.grid
+    rng = np.random.default_rng(17)
+    nf = 2 * len(wl.stations)
+    nodes = rng.integers(0, g.count, args.locate_events)
+    events = []
+    for node in nodes:
+        field = rng.choice(nf, size=40, replace=False).astype(np.int32)
+        events.append(Lc.Picks(field, np.zeros(40), rng.choice([1.0, 0.5, 0.25], 40), rng.normal(0, 0.02, 40),
+                               (0.0, 0.0, 0.0)))
+    tabs = ts.read_f32()  # [field][node] float32, 6.5 GB: the observed times come from the tables
+    for e, node in zip(events, nodes):
+        e.obs = rng.uniform(-2, 2) + tabs[e.field, int(node)].astype(np.float64) + rng.normal(0, 0.05, 40)
+    Lc.grid_search(ts, g, events[:4])  # warm-up
+    t0 = time.perf_counter()
+    lon, lat, dep, torg, node, mis = Lc.grid_search(ts, g, events)
+    sec = time.perf_counter() - t0
+    hit = float(np.mean(node == nodes))
+    out = {"workload": f"{args.locate_events} events x 40 picks over {nf} tables of {g.nx}x{g.ny}x{g.nz}, global "
+                       "search (8000 coarse candidates, then every node within two strides)",
+           "gpu_seconds": round(sec, 4), "events_per_s": round(args.locate_events / sec, 1),
+           "best_node_is_the_source": round(hit, 4)}
+    if cpu:
+        from oracle import oracle as O
+
+        sample = 3
+        t0 = time.perf_counter()
+        for e in events[:sample]:
+            O.locate_grid_search(tabs, g.nx, g.ny, g.nz, (e.field, e.obs, e.weight, e.correction), 8000)
+        per = (time.perf_counter() - t0) / sample
+        out["cpu_seconds_per_event"] = round(per, 3)
+        out["cpu_sample"] = (f"{sample} events, the oracle's numpy restatement on one core (a checker, not an "
+                             "optimised CPU locator)")
+    del tabs
+    log(f"location leg: {out}")
+    return out
+
+
 def consumers_leg(dev, wl, t_obs, r):
     """The multi-RHS consumers of the iteration's system: the resolution diagonal (32 probes, 100 LSQR
     iterations each, one batched solve; the plain FP64 form one probe at a time, timed on 2 probes)
@@ -518,6 +562,7 @@ def main() -> int:
     ap.add_argument("--no-iteration", action="store_true", help="skip the one-iteration pipeline leg")
     ap.add_argument("--uneven-problems", type=int, default=64, help="problems of the uneven-batch leg (0 skips it)")
     ap.add_argument("--serving-requests", type=int, default=16, help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--locate-events", type=int, default=2000, help="events of the location leg (0 skips it)")
     ap.add_argument("--serving-workers", type=int, default=4)
     ap.add_argument("--serving-budget-gb", type=float, default=96.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -610,6 +655,7 @@ def main() -> int:
         log(f"cpu baseline: {cpu}")
 
     lsqr = lsqr_leg(dev, wl, ts, args) if args.lsqr_rays > 0 else None
+    location = location_leg(wl, ts, args, cpu=not args.no_cpu) if args.locate_events > 0 else None
     serving = serving_leg(dev, wl, ts, args) if args.serving_requests > 0 else None
     ts.release()
     it_leg = None if args.no_iteration else iteration_leg(dev, cpu=not args.no_cpu)
@@ -649,6 +695,7 @@ def main() -> int:
         "iteration": it_leg,
         "uneven": uneven,
         "serving": serving,
+        "location": location,
     }
     print(json.dumps(out), flush=True)
     dev.close()
