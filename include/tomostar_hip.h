@@ -253,6 +253,41 @@ int ts_system_build(ts_context *ctx, const ts_grid *grid, const ts_system_opts *
                     const double *sRefS, const double *kRef, int keep_fp64, ts_matrix **out, int64_t layout[8],
                     char *reason, int reason_len);
 
+/* ---------------------------------------------------------------- Q (attenuation) tomography
+ * QTomography.Predict (Attenuation/QTomography.cs:307-312) over rows of resident rays: per row r (arrival
+ * ray_of_row[r]) the predicted t* = q0 * OutsideTime + sum L s q (rows inside the grid: no outside
+ * part) and the path time sum L s (Prepare's Q0 estimate, :232-233), each summed in the row's node
+ * order. slowness, q: [nodes of the rays' inversion grid]; q NULL = q0 at every node. */
+int ts_q_predict(ts_rays *rays, int nrows, const int32_t *ray_of_row, const double *slowness, const double *q,
+                 double q0, double *pred, double *travel_time, char *reason, int reason_len);
+
+/* QTomography.SolveOnce's system (QTomography.cs:318-392) on a plain node grid (no parameter lattice or
+ * adaptive mesh; Laplacian smoothing or unweighted first differences), assembled on the device. */
+typedef struct {
+    int32_t station_terms;       /* StationTerms: one column per station after the nodes */
+    int32_t gradient_smoothing;  /* 0: SmoothingMethod.Laplacian, 1: first differences (no reweighting) */
+    int32_t smoothing_in_nodes;  /* SmoothingScale.Nodes */
+    int32_t nstations;           /* data.Stations.Count */
+    int32_t fixed_scales;        /* 1: scales[] as given (the Gauss-Newton iterations after the first) */
+    double damping, smoothing;   /* relative to the sensitivity scales */
+    double vertical_weight;      /* VerticalSmoothingWeight */
+    double damping_station;      /* DampingStation */
+    double q0;                   /* the reference q = 1 / Q0 */
+} ts_qsystem_opts;
+
+/* Data row r is arrival ray_of_row[r] (the used rows, in order) with station[r], t*[r] and sigma[r]:
+ * entries L s q at the nodes (q = q_old, or q0 everywhere when q_old is NULL), 1 at the station's
+ * column, right-hand side t* - Predict(q0) (q_old NULL) or t* - Predict(q_old) + sum (L s q_old) m_old
+ * (:344-349), all times 1 / sigma; then damping and smoothing of the nodes times the model scale, the
+ * station damping and the zero-sum row (weight 10) times the station scale. q_old and m_old are both
+ * given or both NULL. scales[2] (model, station): the RMS of each block's weighted entries (:356-372),
+ * written on return, or read when fixed_scales. layout[4] = {offset of the station columns (-1: none),
+ * columns, rows, data rows}. */
+int ts_qsystem_build(ts_context *ctx, const ts_grid *grid, const ts_qsystem_opts *opts, ts_rays *rays, int nrows,
+                     const int32_t *ray_of_row, const int32_t *station, const double *tstar, const double *sigma,
+                     const double *slowness, const double *q_old, const double *m_old, int keep_fp64,
+                     double scales[2], ts_matrix **out, int64_t layout[4], char *reason, int reason_len);
+
 /* Lsqr.Solve (Numerics/Lsqr.cs:34-152) / LsqrOpenCl.TrySolve (LsqrOpenCl.cs:244-270) for k (1..64)
  * right-hand sides b[k*m] (RHS-major), each with its own damping damp[k]: minimises
  * ||A x_j - b_j||^2 + damp_j^2 ||x_j||^2 with the reference's stopping rules (atol, btol, conlim).

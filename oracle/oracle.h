@@ -99,6 +99,20 @@ int or_build_system(const or_griddef *g, const or_system_opts *o, int nrows, con
                     int64_t cap_rows, int64_t cap_nnz, int64_t *rs, int *ci, double *va, double *rhs,
                     int64_t dims[4]);
 
+/* --- Q tomography (Attenuation/QTomography.cs): t* rows of a plain node grid --- */
+typedef struct {
+    int station_terms, gradient_smoothing, smoothing_in_nodes, nstations, fixed_scales;
+    double damping, smoothing, vertical_weight, damping_station, q0;
+} or_qsystem_opts;
+
+void or_q_predict(int nrows, const int64_t *row_start, const int *row_nodes, const float *row_len,
+                  const double *slow, const double *q, double q0, double *pred, double *tt);
+int or_build_qsystem(const or_griddef *g, const or_qsystem_opts *o, int nrows, const int64_t *row_start,
+                     const int *row_nodes, const float *row_len, const int *station, const double *tstar,
+                     const double *sigma, const double *slow, const double *q_old, const double *m_old,
+                     int64_t cap_rows, int64_t cap_nnz, int64_t *rs, int *ci, double *va, double *rhs,
+                     int64_t dims[4], double scales[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@ leg import this module, always as the checker, never as the thing measured or sh
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -467,3 +468,230 @@ def build_system(g: GridDef, opts: dict, row_start, nodes, lens, tcalc_grad, pha
         raise RuntimeError("oracle system exceeds its capacity")
     m, cols, nnz, data_rows = (int(x) for x in dims)
     return rs[:m + 1].copy(), ci[:nnz].copy(), va[:nnz].copy(), rhs[:m].copy(), (m, cols, nnz, data_rows)
+
+
+# ------------------------------------------------------------------------------ Q tomography
+
+class QSystemOpts(C.Structure):
+    """The QTomographySettings fields SolveOnce reads (Attenuation/QTomography.cs:12-90)."""
+
+    _fields_ = [("station_terms", C.c_int), ("gradient_smoothing", C.c_int), ("smoothing_in_nodes", C.c_int),
+                ("nstations", C.c_int), ("fixed_scales", C.c_int), ("damping", C.c_double),
+                ("smoothing", C.c_double), ("vertical_weight", C.c_double), ("damping_station", C.c_double),
+                ("q0", C.c_double)]
+
+
+def _declare_q(L):
+    if getattr(L, "_q_declared", False):
+        return
+    L.or_q_predict.restype = None
+    L.or_q_predict.argtypes = [C.c_int, _lp, _ip, _fp, _dp, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+    L.or_build_qsystem.restype = C.c_int
+    L.or_build_qsystem.argtypes = [_G, C.POINTER(QSystemOpts), C.c_int, _lp, _ip, _fp, _ip, _dp, _dp, _dp,
+                                   C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, _lp, _ip, _dp, _dp, _lp, _dp]
+    L._q_declared = True
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def q_predict(row_start, nodes, lens, slow, q, q0):
+    """QTomography.Predict (QTomography.cs:307-312) and the path time sum L s (:232-233) of every row
+    (rows inside the grid); q None: q0 at every node. Returns (pred, tt)."""
+    L = lib()
+    _declare_q(L)
+    rs = np.ascontiguousarray(row_start, np.int64)
+    nr = rs.size - 1
+    pred, tt = np.zeros(max(1, nr)), np.zeros(max(1, nr))
+    qa = None if q is None else np.ascontiguousarray(q, np.float64)
+    L.or_q_predict(nr, rs, np.ascontiguousarray(nodes, np.int32), np.ascontiguousarray(lens, np.float32),
+                   np.ascontiguousarray(slow, np.float64), _vp(qa), float(q0), _vp(pred), _vp(tt))
+    return pred[:nr], tt[:nr]
+
+
+def build_qsystem(g: GridDef, opts: dict, row_start, nodes, lens, station, tstar, sigma, slow, q_old=None, m_old=None,
+                  scales=None):
+    """QTomography.SolveOnce's system restated (QTomography.cs:318-392, plain node grid); the rows are the
+    used ray rows in order. Returns (row_start, column, values, rhs, dims, scales) with dims = (rows,
+    cols, nnz, data rows) and scales = (model, station)."""
+    L = lib()
+    _declare_q(L)
+    o = QSystemOpts(**{k: opts.get(k, 0) for k, _ in QSystemOpts._fields_})
+    o.fixed_scales = 1 if scales is not None else 0
+    n = g.count
+    rs_in = np.ascontiguousarray(row_start, np.int64)
+    nr = rs_in.size - 1
+    nodes = np.ascontiguousarray(nodes, np.int32)
+    cap_rows = nr + 5 * n + o.nstations + 4
+    cap_nnz = int(rs_in[-1]) + 2 * nr + 12 * n + 2 * o.nstations + 8
+    rs = np.zeros(cap_rows + 1, np.int64)
+    ci = np.zeros(cap_nnz, np.int32)
+    va = np.zeros(cap_nnz)
+    rhs = np.zeros(cap_rows)
+    dims = np.zeros(4, np.int64)
+    sc = np.array(scales if scales is not None else (1.0, 1.0), np.float64)
+    qo = None if q_old is None else np.ascontiguousarray(q_old, np.float64)
+    mo = None if m_old is None else np.ascontiguousarray(m_old, np.float64)
+    st = np.ascontiguousarray(station if len(station) else np.zeros(1), np.int32)
+    rc = L.or_build_qsystem(C.byref(g), C.byref(o), nr, rs_in, nodes, np.ascontiguousarray(lens, np.float32), st,
+                            np.ascontiguousarray(tstar, np.float64) if nr else np.zeros(1),
+                            np.ascontiguousarray(sigma, np.float64) if nr else np.ones(1),
+                            np.ascontiguousarray(slow, np.float64), _vp(qo), _vp(mo), cap_rows, cap_nnz, rs, ci, va,
+                            rhs, dims, sc)
+    if rc != 0:
+        raise RuntimeError("oracle Q system exceeds its capacity")
+    m, cols, nnz, data_rows = (int(x) for x in dims)
+    return rs[:m + 1].copy(), ci[:nnz].copy(), va[:nnz].copy(), rhs[:m].copy(), (m, cols, nnz, data_rows), \
+        (float(sc[0]), float(sc[1]))
+
+
+def seq_sum(a) -> float:
+    """A running sum in index order (a foreach accumulation)."""
+    a = np.asarray(a, np.float64)
+    return float(np.cumsum(a)[-1]) if a.size else 0.0
+
+
+def seq_mean(a) -> float:
+    """Enumerable.Average: the running sum over the count."""
+    a = np.asarray(a, np.float64)
+    return seq_sum(a) / a.size
+
+
+class QOracle:
+    """QTomography's host steps (QTomography.cs:199-300 Prepare, :252-289 Solve, :296-405 SolveOnce, :412-432
+    RejectOutliers, :434-505 Run, :520-554 TradeOff) over the C restatement of the system and Lsqr.Solve,
+    for a plain node grid with Laplacian smoothing (or unweighted first differences), on rays given as
+    rows in arrival order. settings: a dict of QTomographySettings fields (q0 as Q0, the rest by name)."""
+
+    DEFAULTS = dict(q0=300.0, estimate_q0=True, damping=2.0, smoothing=10.0, vertical_weight=0.1, smoothing_in_nodes=0,
+                    gradient_smoothing=0, station_terms=True, damping_station=3.0, lsqr_iterations=400,
+                    log_parameterisation=True, nonlinear_iterations=20, q_min=10.0, q_max=5000.0, outlier_mads=4.0)
+
+    def __init__(self, g: GridDef, settings: dict, threads: int = 0):
+        self.g = g
+        self.s = dict(self.DEFAULTS, **settings)
+        self.threads = threads
+
+    # rows: (row_start [nrows+1], nodes, lens) of the arrivals that have rays; arrival[r]: their arrival
+    def prepare(self, row_start, nodes, lens, arrival, tstar, sigma, station, nstations, slow):
+        s = self.s
+        self.rs, self.nodes, self.lens = np.asarray(row_start, np.int64), nodes, lens
+        self.arrival = np.asarray(arrival, np.int64)
+        self.tstar, self.sigma = np.asarray(tstar, np.float64), np.asarray(sigma, np.float64)
+        self.station, self.nstations, self.slow = np.asarray(station, np.int32), nstations, slow
+        q0 = 1 / max(1.0, s["q0"])
+        _, tt = q_predict(self.rs, nodes, lens, slow, None, q0)
+        if s["estimate_q0"] and self.arrival.size:
+            ratios, num, den = [], 0.0, 0.0
+            for r, a in enumerate(self.arrival):
+                t = tt[r]
+                if not (t > 0) or not np.isfinite(self.tstar[a]):
+                    continue
+                ratios.append(self.tstar[a] / t)
+                w = 1 / (self.sigma[a] * self.sigma[a])
+                num += w * self.tstar[a] * t
+                den += w * t * t
+            est = 0.0
+            if s["station_terms"]:
+                if den > 0 and num > 0:
+                    est = num / den
+            elif ratios:
+                ratios.sort()
+                m = len(ratios) // 2
+                est = ratios[m] if len(ratios) % 2 == 1 else 0.5 * (ratios[m - 1] + ratios[m])
+            if est > 0:
+                q0 = min(max(est, 1 / s["q_max"]), 1 / s["q_min"])
+        self.q0 = q0
+        na = self.tstar.size
+        self.rejected = np.ones(na, bool)
+        self.residual = np.full(na, np.nan)
+        pred, _ = q_predict(self.rs, nodes, lens, slow, None, q0)
+        self.start_residual = self.tstar[self.arrival] - pred
+        self.residual[self.arrival] = self.start_residual
+        self.rejected[self.arrival] = False
+        self.rms_before = self.rms()
+
+    def rms(self):
+        used = ~self.rejected & ~np.isnan(self.residual)
+        return math.sqrt(seq_mean(self.residual[used] ** 2)) if used.any() else 0.0
+
+    def _used_rows(self):
+        keep = ~self.rejected[self.arrival]
+        idx = np.flatnonzero(keep)
+        starts, ends = self.rs[idx], self.rs[idx + 1]
+        lens = ends - starts
+        rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        take = np.concatenate([np.arange(a, b) for a, b in zip(starts, ends)]) if idx.size else np.zeros(0, np.int64)
+        return idx, rs, self.nodes[take], self.lens[take]
+
+    def solve_once(self, damping, smoothing, m_old, fixed_scales, x_old):
+        s, g, n = self.s, self.g, self.g.count
+        q0 = self.q0
+        lo, hi = math.log(1 / (s["q_max"] * q0)), math.log(1 / (s["q_min"] * q0))
+        q_old = None if m_old is None else q0 * np.exp(np.clip(m_old, lo, hi))
+        idx, rs, nd, ln = self._used_rows()
+        a = self.arrival[idx]
+        opts = dict(station_terms=int(s["station_terms"]), gradient_smoothing=s["gradient_smoothing"],
+                    smoothing_in_nodes=s["smoothing_in_nodes"], nstations=self.nstations, damping=damping,
+                    smoothing=smoothing, vertical_weight=s["vertical_weight"], damping_station=s["damping_station"],
+                    q0=q0)
+        grs, gci, gva, rhs, dims, scales = build_qsystem(g, opts, rs, nd, ln, self.station[a], self.tstar[a],
+                                                         self.sigma[a], self.slow, q_old, m_old, fixed_scales)
+        self.scales = scales
+        m_rows, cols = dims[0], dims[1]
+        if x_old is not None:
+            rhs = rhs - spmv(grs, gci, gva, x_old, self.threads)
+        x, stop, st, _ = lsqr(m_rows, cols, grs, gci, gva, rhs, 0.0, s["lsqr_iterations"], 1e-6, 1e-6, 1e8,
+                              self.threads)
+        if x_old is not None:
+            x = x + x_old
+        if s["log_parameterisation"]:
+            m_nodes = np.clip(x[:n], lo, hi)
+            q = q0 * np.exp(m_nodes)
+        else:
+            q = np.clip(q0 * (1 + x[:n]), 1 / s["q_max"], 1 / s["q_min"])
+            m_nodes = np.log(q / q0)
+        terms = x[n:n + self.nstations] if s["station_terms"] else np.zeros(0)
+        pred, _ = q_predict(self.rs, self.nodes, self.lens, self.slow, q, q0)
+        self.residual[self.arrival] = self.tstar[self.arrival] - pred - \
+            (terms[self.station[self.arrival]] if terms.size else 0.0)
+        return dict(x=x, q=q, terms=terms, iterations=st["iterations"], stop=stop, dims=dims), m_nodes
+
+    def solve(self, damping, smoothing):
+        s = self.s
+        if not s["log_parameterisation"]:
+            return self.solve_once(damping, smoothing, None, None, None)[0]
+        m = scales = x = None
+        it, change, out = 0, float("nan"), None
+        while it < max(1, s["nonlinear_iterations"]):
+            out, m_nodes = self.solve_once(damping, smoothing, m, scales, x)
+            x = out["x"]
+            scales = scales or self.scales
+            change = math.inf if m is None else math.sqrt(seq_mean((m_nodes - m) ** 2))
+            m = m_nodes
+            it += 1
+            if change < 1e-3:
+                break
+        self.gn = (it, change)
+        return out
+
+    def reject_outliers(self):
+        keep = ~self.rejected
+        fitted = np.sort(self.residual[keep])
+        med = fitted[fitted.size // 2] if fitted.size else 0.0
+        mad = 1.4826 * (np.sort(np.abs(fitted - med))[fitted.size // 2] if fitted.size else 0.0)
+        sig = self.sigma[keep]
+        limit = max(self.s["outlier_mads"] * mad, 2 * (seq_mean(sig) if sig.size else 0.002))
+        bad = keep & (np.abs(self.residual - med) > limit)
+        self.rejected |= bad
+        return int(bad.sum())
+
+    def run(self):
+        out = self.solve(self.s["damping"], self.s["smoothing"])
+        rejected = self.reject_outliers()
+        if rejected > 0:
+            out = self.solve(self.s["damping"], self.s["smoothing"])
+        out["rejected"] = rejected
+        out["rms_after"] = self.rms()
+        return out

@@ -236,3 +236,105 @@ int or_build_system(const or_griddef *g, const or_system_opts *o, int nrows, con
     dims[3] = data_rows;
     return b.overflow ? -1 : 0;
 }
+
+/* ---------------------------------------------------------------- Q tomography
+ * QTomography.Predict (Attenuation/QTomography.cs:307-312) and the path time of Prepare's Q0 estimate
+ * (:232-233): per row t* = q0 * OutsideTime + sum L s q and T = OutsideTime + sum L s, in the row's
+ * order. The rows lie inside the grid (OutsideTime 0). q NULL: q0 at every node. */
+void or_q_predict(int nrows, const int64_t *row_start, const int *row_nodes, const float *row_len,
+                  const double *slow, const double *q, double q0, double *pred, double *tt)
+{
+    for (int r = 0; r < nrows; r++) {
+        double t = q0 * 0.0, T = 0.0;
+        for (int64_t k = row_start[r]; k < row_start[r + 1]; k++) {
+            int nd = row_nodes[k];
+            t += (double)row_len[k] * slow[nd] * (q ? q[nd] : q0);
+            T += (double)row_len[k] * slow[nd];
+        }
+        if (pred) pred[r] = t;
+        if (tt) tt[r] = T;
+    }
+}
+
+/* QTomography.SolveOnce's system (Attenuation/QTomography.cs:318-392) on a plain node grid: columns
+ * [nodes | stations (StationTerms)]; per used row (in the order given) the node entries L s q (q_old,
+ * or q0 everywhere when q_old is NULL), the station entry 1 and the data right-hand side - the start
+ * residual t* - Predict(q0) without m_old, else t* - Predict(q_old) + sum (L s q_old) m_old (:344-349)
+ * - all times 1 / sigma; the sensitivity scales (:365-372: RMS of the weighted entries of each block,
+ * or the given ones when fixed_scales); damping and smoothing of the nodes times the model scale, the
+ * station damping and the zero-sum row (weight 10) times the station scale (:379-392).
+ * scales[2] in/out (model, station). Returns 0, or -1 when the output arrays are too small;
+ * dims = {rows, cols, nnz, data rows}. */
+int or_build_qsystem(const or_griddef *g, const or_qsystem_opts *o, int nrows, const int64_t *row_start,
+                     const int *row_nodes, const float *row_len, const int *station, const double *tstar,
+                     const double *sigma, const double *slow, const double *q_old, const double *m_old,
+                     int64_t cap_rows, int64_t cap_nnz, int64_t *rs, int *ci, double *va, double *rhs,
+                     int64_t dims[4], double scales[2])
+{
+    const int n = g->nx * g->ny * g->nz;
+    const int off_st = o->station_terms ? n : -1;
+    const int cols = n + (o->station_terms ? o->nstations : 0);
+    builder b = {0, 0, cap_rows, cap_nnz, rs, ci, va, rhs, 0};
+    rs[0] = 0;
+    int maxlen = 1;
+    for (int r = 0; r < nrows; r++)
+        if (row_start[r + 1] - row_start[r] + 1 > maxlen) maxlen = (int)(row_start[r + 1] - row_start[r] + 1);
+    int *c = malloc(sizeof(int) * maxlen);
+    double *v = malloc(sizeof(double) * maxlen);
+    double sum_model = 0, sum_station = 0;
+    int64_t count_model = 0, count_station = 0;
+    for (int r = 0; r < nrows; r++) {
+        int len = 0;
+        for (int64_t k = row_start[r]; k < row_start[r + 1]; k++) {
+            int nd = row_nodes[k];
+            c[len] = nd;
+            v[len] = (double)row_len[k] * slow[nd] * (q_old ? q_old[nd] : o->q0);
+            len++;
+        }
+        double d;
+        if (!q_old) {
+            double p = 0;
+            or_q_predict(1, row_start + r, row_nodes, row_len, slow, NULL, o->q0, &p, NULL);
+            d = tstar[r] - p;
+        } else {
+            double p = 0;
+            or_q_predict(1, row_start + r, row_nodes, row_len, slow, q_old, o->q0, &p, NULL);
+            d = tstar[r] - p;
+            for (int k = 0; k < len; k++) d += v[k] * m_old[c[k]];
+        }
+        if (off_st >= 0) { c[len] = off_st + station[r]; v[len] = 1; len++; }
+        for (int k = 0; k < len; k++) {
+            double w = v[k] / sigma[r];
+            if (c[k] < n) { sum_model += w * w; count_model++; }
+            else { sum_station += w * w; count_station++; }
+        }
+        add_row(&b, len, c, v, d, 1 / sigma[r]);
+    }
+    free(c);
+    free(v);
+    const int64_t data_rows = b.rows;
+    double sm = count_model > 0 ? sqrt(sum_model / count_model) : 1;
+    double ss = count_station > 0 ? sqrt(sum_station / count_station) : 1;
+    if (o->fixed_scales) { sm = scales[0]; ss = scales[1]; }
+    scales[0] = sm;
+    scales[1] = ss;
+    add_damping(&b, 0, n, o->damping * sm);
+    if (o->gradient_smoothing)
+        add_gradient(&b, g, 0, o->smoothing * sm, o->vertical_weight, o->smoothing_in_nodes, NULL);
+    else
+        add_laplacian(&b, g, 0, o->smoothing * sm, o->vertical_weight, o->smoothing_in_nodes, NULL);
+    if (off_st >= 0) {
+        add_damping(&b, off_st, o->nstations, o->damping_station * ss);
+        int *zc = malloc(sizeof(int) * (o->nstations + 1));
+        double *zv = malloc(sizeof(double) * (o->nstations + 1));
+        for (int s = 0; s < o->nstations; s++) { zc[s] = off_st + s; zv[s] = 1; }
+        add_row(&b, o->nstations, zc, zv, 0, 10 * ss);
+        free(zc);
+        free(zv);
+    }
+    dims[0] = b.rows;
+    dims[1] = cols;
+    dims[2] = b.nnz;
+    dims[3] = data_rows;
+    return b.overflow ? -1 : 0;
+}

@@ -340,6 +340,115 @@ __global__ void sys_rs32(const int64_t *__restrict__ rs, int64_t n, int32_t *__r
         out[q] = (int32_t)rs[q];
 }
 
+// ---- Q tomography's t* rows (QTomography.SolveOnce, Attenuation/QTomography.cs:318-364)
+
+struct QArgs {
+    const int32_t *nodes;      // ray CSR (arrival order)
+    const float *lengths;
+    const int64_t *ray_start;  // [nrays + 1]
+    const int32_t *ray_of_row;
+    const double *slow;        // slowness of the phase inverted
+    const double *q;           // q = 1/Q the rows are formed at; null: q0 at every node
+    double q0;
+    int nrows;
+};
+
+// dt*/dq of one node: (L s) q, the kernel at q (:337; Predict's term, :310)
+__device__ inline double q_entry(const QArgs &A, int64_t e, int nd)
+{
+    return (double)A.lengths[e] * A.slow[nd] * (A.q ? A.q[nd] : A.q0);
+}
+
+// One thread per row, its entries in order: the predicted t* (Predict, :307-312: q0 times the time
+// outside the grid, none here, plus sum L s q), the path time sum L s (:232-233), the entries AddRow
+// keeps (not exactly zero), the row's sum of squared weighted entries (v / sigma)^2 for the model
+// scale (:356-361), and the data right-hand side t* - Predict + sum (L s q) m (:344-349; without m,
+// the start residual). Every output may be null.
+__global__ __launch_bounds__(256) void q_rows(QArgs A, const double *__restrict__ tstar, const double *__restrict__ m,
+                                              const double *__restrict__ sigma, double *__restrict__ pred,
+                                              double *__restrict__ tt, int32_t *__restrict__ count,
+                                              double *__restrict__ wsum, double *__restrict__ rhs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.nrows) return;
+    const int ray = A.ray_of_row[r];
+    const int64_t b = A.ray_start[ray], e = A.ray_start[ray + 1];
+    const double sg = sigma ? sigma[r] : 1.0;
+    double p = A.q0 * 0.0, t = 0.0, s2 = 0.0;
+    int c = 0;
+    for (int64_t k = b; k < e; k++) {
+        const int nd = A.nodes[k];
+        const double ls = (double)A.lengths[k] * A.slow[nd];
+        const double v = ls * (A.q ? A.q[nd] : A.q0);
+        p += v;
+        t += ls;
+        c += v != 0;
+        const double w = v / sg;
+        s2 += w * w;
+    }
+    if (pred) pred[r] = p;
+    if (tt) tt[r] = t;
+    if (count) count[r] = c;
+    if (wsum) wsum[r] = s2;
+    if (rhs) {
+        double d = tstar[r] - p;
+        if (m)
+            for (int64_t k = b; k < e; k++) {
+                const int nd = A.nodes[k];
+                d += q_entry(A, k, nd) * m[nd];
+            }
+        rhs[r] = d;
+    }
+}
+
+// entries per data row: the kept node entries, plus the station term's
+__global__ void q_counts(const int32_t *__restrict__ count, int nrows, int station_terms, int64_t *__restrict__ rowcnt)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrows) rowcnt[r] = count[r] + station_terms;
+}
+
+// The entries of each data row at its row start (one wavefront per row): the node entries in node
+// order without exact zeros, the station term 1 (:351), all times 1 / sigma, and the right-hand side
+// times 1 / sigma (CsrBuilder.AddRow, SparseMatrix.cs:128-140).
+__global__ __launch_bounds__(256) void q_fill(QArgs A, const int32_t *__restrict__ station,
+                                              const double *__restrict__ sigma, const double *__restrict__ rhs_in,
+                                              int off_st, const int64_t *__restrict__ rs, int32_t *__restrict__ ci,
+                                              double *__restrict__ va, double *__restrict__ rhs)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= A.nrows) return;
+    const double w = 1 / sigma[r];
+    const int ray = A.ray_of_row[r];
+    const int64_t b = A.ray_start[ray], e = A.ray_start[ray + 1];
+    int64_t at = rs[r];
+    for (int64_t q0 = b; q0 < e; q0 += 64) {
+        const int64_t q = q0 + lane;
+        double v = 0;
+        int nd = 0;
+        if (q < e) {
+            nd = A.nodes[q];
+            v = q_entry(A, q, nd);
+        }
+        const bool k = q < e && v != 0;
+        const uint64_t mask = __ballot(k);
+        if (k) {
+            const int pos = __popcll(mask & ((1ull << lane) - 1));
+            ci[at + pos] = nd;
+            va[at + pos] = v * w;
+        }
+        at += __popcll(mask);
+    }
+    if (lane == 0) {
+        if (off_st >= 0) {
+            ci[at] = off_st + station[r];
+            va[at] = 1.0 * w;
+        }
+        rhs[r] = rhs_in[r] * w;
+    }
+}
+
 unsigned blocks(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
 
 // AxisWeights (Observations.cs:390-397): spacings at (Nx/2, Ny/2, 0) (SphericalGrid.Spacing, :152-156)
@@ -389,6 +498,59 @@ void upload(dbuf<T> &d, const T *h, size_t n, hipStream_t st)
 {
     d.alloc(std::max<size_t>(1, n));
     if (n) TS_HIP(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+// Completes a system whose data rows come first: their row counts (count_data writes rows
+// [0, data rows) of the count array), the blocks' counts, one scan to row starts, the data rows'
+// entries (fill_data) and the blocks', int32 row starts, the right-hand side to the host, and the
+// stored orders and transpose (matrix_finish), the nodes of the blocks starting at node_blocks stored
+// in tiles.
+template <typename CountData, typename FillData>
+std::unique_ptr<ts_matrix> assemble(ts_context *ctx, hipStream_t st, const Grid &g, const RegArgs &G,
+                                    const std::vector<Block> &regs, int64_t m, int64_t col, bool keep_fp64,
+                                    std::initializer_list<int> node_blocks, CountData count_data, FillData fill_data)
+{
+    if (m >= INT32_MAX) fail(TS_DECLINED, "%lld rows exceed the int32 index range", (long long)m);
+    dbuf<int64_t> rowcnt(m + 1), rs64(m + 1);
+    TS_HIP(hipMemsetAsync(rowcnt.p, 0, (m + 1) * 8, st));
+    count_data(rowcnt.p);
+    for (const Block &B : regs) {
+        const int64_t threads = B.kind == kDamp || B.kind == kGradient ? B.rows : B.kind == kZeroSum ? 1 : B.count;
+        sys_reg_count<<<blocks(threads, 256), 256, 0, st>>>(B, G, rowcnt.p);
+    }
+    TS_HIP(hipGetLastError());
+    {
+        size_t tb = 0;
+        TS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, rowcnt.p, rs64.p, m + 1, st));
+        dbuf<unsigned char> tmp(std::max<size_t>(1, tb));
+        TS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, rowcnt.p, rs64.p, m + 1, st));
+    }
+    int64_t nnz = 0;
+    TS_HIP(hipMemcpyAsync(&nnz, rs64.p + m, 8, hipMemcpyDeviceToHost, st));
+    TS_HIP(hipStreamSynchronize(st));
+    if (nnz >= INT32_MAX) fail(TS_DECLINED, "%lld non-zeros exceed the int32 index range", (long long)nnz);
+
+    std::unique_ptr<ts_matrix> M(new ts_matrix);
+    M->ctx = ctx;
+    M->rows = (int)m;
+    M->cols = (int)col;
+    M->nnz = nnz;
+    M->rs.alloc(m + 1);
+    M->ci.alloc(std::max<int64_t>(1, nnz));
+    dbuf<double> va(std::max<int64_t>(1, nnz)), rhs(std::max<int64_t>(1, m));
+    fill_data(rs64.p, M->ci.p, va.p, rhs.p);
+    for (const Block &B : regs) {
+        const int64_t threads = B.kind == kDamp ? B.rows : B.kind == kZeroSum ? std::max(1, B.count) : B.count;
+        sys_reg_fill<<<blocks(threads, 256), 256, 0, st>>>(B, G, rs64.p, M->ci.p, va.p, rhs.p);
+    }
+    sys_rs32<<<blocks(std::min<int64_t>(m + 1, 1 << 22), 256), 256, 0, st>>>(rs64.p, m + 1, M->rs.p);
+    TS_HIP(hipGetLastError());
+    M->rhs.resize(m);
+    if (m > 0) TS_HIP(hipMemcpyAsync(M->rhs.data(), rhs.p, m * 8, hipMemcpyDeviceToHost, st));
+    TS_HIP(hipStreamSynchronize(st));
+    const std::vector<int32_t> order = stored_columns(g, (int)col, node_blocks);
+    matrix_finish(*M, std::move(va), keep_fp64, order.empty() ? nullptr : order.data());
+    return M;
 }
 
 }  // namespace
@@ -534,54 +696,186 @@ int ts_system_build(ts_context *ctx, const ts_grid *grid, const ts_system_opts *
             regs.push_back(Block{kZeroSum, off, o->nstations, m, 1, 10.0, nullptr});
             m += 1;
         }
-        if (m >= INT32_MAX) fail(TS_DECLINED, "%lld rows exceed the int32 index range", (long long)m);
-
-        // row counts -> row starts
-        dbuf<int64_t> rowcnt(m + 1), rs64(m + 1);
-        TS_HIP(hipMemsetAsync(rowcnt.p, 0, (m + 1) * 8, st));
-        if (data) sys_scatter_counts<<<blocks(nrows, 256), 256, 0, st>>>(d_count.p, d_dest.p, nrows, rowcnt.p);
-        for (const Block &B : regs) {
-            const int64_t threads = B.kind == kDamp || B.kind == kGradient ? B.rows : B.kind == kZeroSum ? 1 : B.count;
-            sys_reg_count<<<blocks(threads, 256), 256, 0, st>>>(B, G, rowcnt.p);
-        }
-        TS_HIP(hipGetLastError());
-        {
-            size_t tb = 0;
-            TS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, rowcnt.p, rs64.p, m + 1, st));
-            dbuf<unsigned char> tmp(std::max<size_t>(1, tb));
-            TS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, rowcnt.p, rs64.p, m + 1, st));
-        }
-        int64_t nnz = 0;
-        TS_HIP(hipMemcpyAsync(&nnz, rs64.p + m, 8, hipMemcpyDeviceToHost, st));
-        TS_HIP(hipStreamSynchronize(st));
-        if (nnz >= INT32_MAX) fail(TS_DECLINED, "%lld non-zeros exceed the int32 index range", (long long)nnz);
-
-        std::unique_ptr<ts_matrix> M(new ts_matrix);
-        M->ctx = ctx;
-        M->rows = (int)m;
-        M->cols = (int)col;
-        M->nnz = nnz;
-        M->rs.alloc(m + 1);
-        M->ci.alloc(std::max<int64_t>(1, nnz));
-        dbuf<double> va(std::max<int64_t>(1, nnz)), rhs(std::max<int64_t>(1, m));
-        if (data) sys_data_fill<<<blocks(nrows, 4), 256, 0, st>>>(A, d_count.p, d_dest.p, rs64.p, M->ci.p, va.p, rhs.p);
-        for (const Block &B : regs) {
-            const int64_t threads = B.kind == kDamp ? B.rows : B.kind == kZeroSum ? std::max(1, B.count) : B.count;
-            sys_reg_fill<<<blocks(threads, 256), 256, 0, st>>>(B, G, rs64.p, M->ci.p, va.p, rhs.p);
-        }
-        sys_rs32<<<blocks(std::min<int64_t>(m + 1, 1 << 22), 256), 256, 0, st>>>(rs64.p, m + 1, M->rs.p);
-        TS_HIP(hipGetLastError());
-        M->rhs.resize(m);
-        if (m > 0) TS_HIP(hipMemcpyAsync(M->rhs.data(), rhs.p, m * 8, hipMemcpyDeviceToHost, st));
-        TS_HIP(hipStreamSynchronize(st));
-        {
-            const std::vector<int32_t> order = stored_columns(g, (int)col, {(int)offP, (int)offS});
-            matrix_finish(*M, std::move(va), keep_fp64 != 0, order.empty() ? nullptr : order.data());
-        }
+        std::unique_ptr<ts_matrix> M = assemble(
+            ctx, st, g, G, regs, m, col, keep_fp64 != 0, {(int)offP, (int)offS},
+            [&](int64_t *rowcnt) {
+                if (data) sys_scatter_counts<<<blocks(nrows, 256), 256, 0, st>>>(d_count.p, d_dest.p, nrows, rowcnt);
+            },
+            [&](const int64_t *rs64, int32_t *ci, double *va, double *rhs) {
+                if (data) sys_data_fill<<<blocks(nrows, 4), 256, 0, st>>>(A, d_count.p, d_dest.p, rs64, ci, va, rhs);
+            });
         M->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (layout) {
             const int64_t L[8] = {offP, offS, offH, offCP, offCS, col, m, data_rows};
             std::copy(L, L + 8, layout);
+        }
+        *out = M.release();
+        return TS_OK;
+    });
+}
+
+int ts_q_predict(ts_rays *rays, int nrows, const int32_t *ray_of_row, const double *slowness, const double *q,
+                 double q0, double *pred, double *travel_time, char *reason, int reason_len)
+{
+    if (!rays || nrows < 0 || !slowness || (nrows > 0 && !ray_of_row) || !(q0 > 0)) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    return guarded(reason, reason_len, [&]() -> int {
+        ts_context *ctx = rays->ctx;
+        TS_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const Grid g = make_grid(rays->inversion);
+        const int n = g.nx * g.ny * g.nz;
+        for (int r = 0; r < nrows; r++)
+            if (ray_of_row[r] < 0 || ray_of_row[r] >= rays->nrays || rays->counts[ray_of_row[r]] < 0)
+                fail(TS_EINVAL, "row %d names arrival %d, which has no ray", r, ray_of_row[r]);
+        if (nrows == 0) return TS_OK;
+        dbuf<int64_t> d_rstart;
+        dbuf<int32_t> d_ray;
+        dbuf<double> d_slow, d_q, d_pred(nrows), d_tt(nrows);
+        upload(d_rstart, rays->row_start.data(), rays->row_start.size(), st);
+        upload(d_ray, ray_of_row, nrows, st);
+        upload(d_slow, slowness, n, st);
+        if (q) upload(d_q, q, n, st);
+        QArgs A{rays->nodes.p, rays->lengths.p, d_rstart.p, d_ray.p, d_slow.p, q ? d_q.p : nullptr, q0, nrows};
+        q_rows<<<blocks(nrows, 256), 256, 0, st>>>(A, nullptr, nullptr, nullptr, d_pred.p, d_tt.p, nullptr, nullptr,
+                                                   nullptr);
+        TS_HIP(hipGetLastError());
+        if (pred) TS_HIP(hipMemcpyAsync(pred, d_pred.p, nrows * 8, hipMemcpyDeviceToHost, st));
+        if (travel_time) TS_HIP(hipMemcpyAsync(travel_time, d_tt.p, nrows * 8, hipMemcpyDeviceToHost, st));
+        TS_HIP(hipStreamSynchronize(st));
+        return TS_OK;
+    });
+}
+
+int ts_qsystem_build(ts_context *ctx, const ts_grid *grid, const ts_qsystem_opts *o, ts_rays *rays, int nrows,
+                     const int32_t *ray_of_row, const int32_t *station, const double *tstar, const double *sigma,
+                     const double *slowness, const double *q_old, const double *m_old, int keep_fp64,
+                     double scales[2], ts_matrix **out, int64_t layout[4], char *reason, int reason_len)
+{
+    if (out) *out = nullptr;
+    if (!ctx || !grid || !o || !rays || nrows < 0 || !out || !slowness || !scales || !(o->q0 > 0) ||
+        (nrows > 0 && (!ray_of_row || !tstar || !sigma || (o->station_terms && !station))) ||
+        (!q_old) != (!m_old) || o->nstations < 0 || !grid_valid(*grid)) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    return guarded(reason, reason_len, [&]() -> int {
+        TS_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        auto t0 = std::chrono::steady_clock::now();
+        const Grid g = make_grid(*grid);
+        const int n = g.nx * g.ny * g.nz;
+        if (rays->ctx != ctx) fail(TS_EINVAL, "the rays belong to another context");
+        if (rays->inversion.nx != g.nx || rays->inversion.ny != g.ny || rays->inversion.nz != g.nz)
+            fail(TS_EINVAL, "the rays were traced on another inversion grid");
+        for (int r = 0; r < nrows; r++) {
+            if (ray_of_row[r] < 0 || ray_of_row[r] >= rays->nrays || rays->counts[ray_of_row[r]] < 0)
+                fail(TS_EINVAL, "row %d names arrival %d, which has no ray", r, ray_of_row[r]);
+            if (!(sigma[r] > 0)) fail(TS_EINVAL, "row %d: sigma %g", r, sigma[r]);
+            if (o->station_terms && (station[r] < 0 || station[r] >= o->nstations))
+                fail(TS_EINVAL, "row %d: station %d of %d", r, station[r], o->nstations);
+        }
+        // columns (:326-328): the nodes, then one term per station
+        const int off_st = o->station_terms ? n : -1;
+        const int64_t col = (int64_t)n + (o->station_terms ? o->nstations : 0);
+        if (col >= INT32_MAX) fail(TS_DECLINED, "%lld columns exceed the int32 index range", (long long)col);
+
+        const int nr = std::max(1, nrows);
+        dbuf<int64_t> d_rstart;
+        dbuf<int32_t> d_ray, d_station, d_count(nr);
+        dbuf<double> d_slow, d_q, d_m, d_tstar, d_sigma, d_wsum(nr), d_rhs(nr);
+        upload(d_rstart, rays->row_start.data(), rays->row_start.size(), st);
+        upload(d_ray, ray_of_row, nrows, st);
+        std::vector<int32_t> zeros;
+        if (!o->station_terms) zeros.assign(nrows, 0);
+        upload(d_station, o->station_terms ? station : zeros.data(), nrows, st);
+        upload(d_tstar, tstar, nrows, st);
+        upload(d_sigma, sigma, nrows, st);
+        upload(d_slow, slowness, n, st);
+        if (q_old) {
+            upload(d_q, q_old, n, st);
+            upload(d_m, m_old, n, st);
+        }
+        QArgs A{rays->nodes.p, rays->lengths.p, d_rstart.p, d_ray.p, d_slow.p, q_old ? d_q.p : nullptr, o->q0, nrows};
+        if (nrows > 0) {
+            q_rows<<<blocks(nrows, 256), 256, 0, st>>>(A, d_tstar.p, q_old ? d_m.p : nullptr, d_sigma.p, nullptr,
+                                                       nullptr, d_count.p, d_wsum.p, d_rhs.p);
+            TS_HIP(hipGetLastError());
+        }
+
+        // the sensitivity scales (:356-372): the RMS of the weighted entries of each block. The station
+        // block's sum runs over the rows in order, as the reference's; the model block's sums each row's
+        // terms in order and then the rows in order.
+        std::vector<double> wsum(nrows);
+        if (nrows > 0) {
+            TS_HIP(hipMemcpyAsync(wsum.data(), d_wsum.p, nrows * 8, hipMemcpyDeviceToHost, st));
+            TS_HIP(hipStreamSynchronize(st));
+        }
+        double sum_model = 0, sum_station = 0;
+        int64_t count_model = 0, count_station = 0;
+        for (int r = 0; r < nrows; r++) {
+            const int ray = ray_of_row[r];
+            sum_model += wsum[r];
+            count_model += rays->row_start[ray + 1] - rays->row_start[ray];
+            if (o->station_terms) {
+                const double w = 1.0 / sigma[r];
+                sum_station += w * w;
+                count_station++;
+            }
+        }
+        double sm = count_model > 0 ? std::sqrt(sum_model / (double)count_model) : 1;
+        double ss = count_station > 0 ? std::sqrt(sum_station / (double)count_station) : 1;
+        if (o->fixed_scales) { sm = scales[0]; ss = scales[1]; }
+
+        // regularisation (:373-392), plain node grid
+        double aw[3];
+        axis_weights(g, o->vertical_weight, o->smoothing_in_nodes != 0, aw);
+        RegArgs G{g.nx, g.ny, g.nz, {aw[0], aw[1], aw[2]}, {std::sqrt(aw[0]), std::sqrt(aw[1]), std::sqrt(aw[2])}};
+        std::vector<Block> regs;
+        int64_t m = nrows;
+        const double ld = o->damping * sm, ls = o->smoothing * sm;
+        if (ld > 0) { regs.push_back(Block{kDamp, 0, n, m, n, ld, nullptr}); m += n; }
+        if (ls > 0) {
+            if (o->gradient_smoothing) {
+                int64_t rows = 0;
+                if (aw[0] > 0) rows += (int64_t)(g.nx - 1) * g.ny * g.nz;
+                if (aw[1] > 0) rows += (int64_t)g.nx * (g.ny - 1) * g.nz;
+                if (aw[2] > 0) rows += (int64_t)g.nx * g.ny * (g.nz - 1);
+                regs.push_back(Block{kGradient, 0, n, m, rows, ls, nullptr});
+                m += rows;
+            } else {
+                regs.push_back(Block{kLaplacian, 0, n, m, n, ls, nullptr});
+                m += n;
+            }
+        }
+        if (off_st >= 0) {
+            const double lst = o->damping_station * ss;
+            if (lst > 0 && o->nstations > 0) {
+                regs.push_back(Block{kDamp, off_st, o->nstations, m, o->nstations, lst, nullptr});
+                m += o->nstations;
+            }
+            regs.push_back(Block{kZeroSum, off_st, o->nstations, m, 1, 10 * ss, nullptr});
+            m += 1;
+        }
+        std::unique_ptr<ts_matrix> M = assemble(
+            ctx, st, g, G, regs, m, col, keep_fp64 != 0, {0},
+            [&](int64_t *rowcnt) {
+                if (nrows > 0)
+                    q_counts<<<blocks(nrows, 256), 256, 0, st>>>(d_count.p, nrows, o->station_terms ? 1 : 0, rowcnt);
+            },
+            [&](const int64_t *rs64, int32_t *ci, double *va, double *rhs) {
+                if (nrows > 0)
+                    q_fill<<<blocks(nrows, 4), 256, 0, st>>>(A, d_station.p, d_sigma.p, d_rhs.p, off_st, rs64, ci, va,
+                                                            rhs);
+            });
+        M->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        scales[0] = sm;
+        scales[1] = ss;
+        if (layout) {
+            const int64_t L[4] = {off_st, col, m, nrows};
+            std::copy(L, L + 4, layout);
         }
         *out = M.release();
         return TS_OK;
