@@ -200,7 +200,55 @@ def iteration_leg(dev, cpu: bool):
     if cpu:
         out["cpu_seconds"] = cpu_baseline_iteration(wl, r)
         out["consumers"]["cpu_seconds"] = cpu_baseline_consumers(r, out["consumers"])
+    out["q_tomography"] = q_leg(dev, wl, cpu)
     log(f"iteration leg: {out}")
+    return out
+
+
+def q_leg(dev, wl, cpu: bool):
+    """Q tomography from t* on the same case: P rays of the 3750 event x station pairs traced in the
+    true model, t* of a Q model (ln Q checkerboard +-0.3 around 250, a Q/3 body) with 4 ms noise, and
+    QTomography.Run with the reference's defaults (Q0 from the data, ln(q/q0) by up to 20 Gauss-Newton
+    iterations, 400 LSQR iterations with atol = btol = 1e-6, outliers rejected and solved again). The
+    device run in the fast mixed LSQR form and in the plain FP64 one; the oracle's Run on the host cores."""
+    from tomostar_amd import qtomography as QT
+
+    c = QT.q_case(dev, wl)
+    s = QT.QTomographySettings()
+    QT.QTomography(dev, wl.grid, QT.QTomographySettings(lsqr_iterations=20, nonlinear_iterations=1),
+                   lsqr_mode="fast_mixed").run(c.rays, c.data, c.slowness)  # warm-up
+    res = {}
+    for mode in ("fast_mixed", "plain_fp64"):
+        t0 = time.perf_counter()
+        res[mode] = QT.QTomography(dev, wl.grid, s, lsqr_mode=mode).run(c.rays, c.data, c.slowness)
+        res[mode].seconds["total"] = time.perf_counter() - t0
+    f, p = res["fast_mixed"], res["plain_fp64"]
+    out = {"workload": "P t* of 3750 paths on 61x61x31, reference defaults (up to 20 Gauss-Newton iterations, "
+                       "400 LSQR iterations, atol = btol = 1e-6)",
+           "gpu_seconds": {k: round(v, 4) for k, v in f.seconds.items()},
+           "gauss_newton_iterations": f.gauss_newton[0], "rejected": f.rejected,
+           "reference_q": round(f.reference_q, 2), "rms_ms": [round(f.rms_before * 1e3, 3), round(f.rms_after * 1e3, 3)],
+           "plain_fp64_gpu_seconds": round(p.seconds["total"], 4),
+           "max_rel_q_difference_fast_vs_plain": float(np.max(np.abs(f.q / p.q - 1)))}
+    if cpu:
+        from oracle import oracle as O
+
+        g = wl.grid
+        og = O.griddef(g.min_lon, g.max_lon, g.min_lat, g.max_lat, g.min_depth, g.max_depth, g.nx, g.ny, g.nz)
+        cores = min(os.cpu_count() or 1, 16)
+        rows = np.flatnonzero(c.rays.counts >= 0)
+        starts, ends = c.rays.row_start[rows], c.rays.row_start[rows + 1]
+        rs = np.concatenate([[0], np.cumsum(ends - starts)]).astype(np.int64)
+        take = np.concatenate([np.arange(a, b) for a, b in zip(starts, ends)])
+        t0 = time.perf_counter()
+        orc = O.QOracle(og, {}, threads=cores)
+        orc.prepare(rs, c.rays.nodes[take], c.rays.lengths[take], rows, c.data.tstar, c.data.sigma, c.data.station,
+                    c.data.nstations, c.slowness)
+        orc.run()
+        out["cpu_seconds"] = round(time.perf_counter() - t0, 3)
+        out["cpu_cores"] = cores
+        out["cpu_sample"] = "the whole Run, the oracle's restatement (C system, LSQR with 16-thread SpMV)"
+    c.rays.release()
     return out
 
 
