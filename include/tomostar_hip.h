@@ -288,6 +288,23 @@ int ts_qsystem_build(ts_context *ctx, const ts_grid *grid, const ts_qsystem_opts
                      const double *slowness, const double *q_old, const double *m_old, int keep_fp64,
                      double scales[2], ts_matrix **out, int64_t layout[4], char *reason, int reason_len);
 
+/* ---------------------------------------------------------------- formal resolution
+ * FormalResolution.Compute (Tomography/FormalResolution.cs:108-333) on the device: for a weighted system
+ * (CSR in the given order, FP64 values; rows below data_rows are data, the rest regularisation) and
+ * ngroups groups of columns to separate out (group g is group_columns[group_start[g] ..
+ * group_start[g+1]): the 4 hypocentre columns of an event), the kept columns (kept_out[*nkept], in
+ * order), their resolution diagonal R_ii, posterior standard deviation sqrt(C_ii) and, when places
+ * (xyz[3 cols], geocentric km) and place_block[cols] (-1: no place) are given, the resolution length
+ * in km (NaN without a place). Output arrays hold cols entries. More than max_unknowns kept columns
+ * declines (> 0); a row coupling two groups, a regularisation row coupling kept and separated columns,
+ * or a reduced normal matrix that is not positive definite are errors (< 0), with the reference's
+ * reasons. seconds: wall time of the call. */
+int ts_formal_resolution(ts_context *ctx, int rows, int cols, const int32_t *row_start, const int32_t *column,
+                         const double *values, int data_rows, int ngroups, const int32_t *group_start,
+                         const int32_t *group_columns, int max_unknowns, const double *places,
+                         const int32_t *place_block, int32_t *kept_out, int32_t *nkept, double *diagonal,
+                         double *stddev, double *length_km, double *seconds, char *reason, int reason_len);
+
 /* Lsqr.Solve (Numerics/Lsqr.cs:34-152) / LsqrOpenCl.TrySolve (LsqrOpenCl.cs:244-270) for k (1..64)
  * right-hand sides b[k*m] (RHS-major), each with its own damping damp[k]: minimises
  * ||A x_j - b_j||^2 + damp_j^2 ||x_j||^2 with the reference's stopping rules (atol, btol, conlim).

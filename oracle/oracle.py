@@ -695,3 +695,73 @@ class QOracle:
         out["rejected"] = rejected
         out["rms_after"] = self.rms()
         return out
+
+
+# ------------------------------------------------------------------------------ formal resolution
+
+def formal_resolution(rows, cols, row_start, column, values, data_rows, eliminate=(), max_unknowns=20000,
+                      places=None):
+    """FormalResolution.Compute (Tomography/FormalResolution.cs:108-333) restated on dense numpy for small
+    systems: N = A^T A, Gamma = the regularisation rows' part on the kept columns, the kept block reduced
+    by the Schur complement of each eliminated group (its live columns, N_gg diagonal > 0; a group whose
+    block is not positive definite is left out, as the reference's SmallCholesky failure does), S = L L^T,
+    C_ii = (S^-1)_ii, R_ii = 1 - (S^-1 Gamma)_ii and, with places (xyz[cols, 3], block[cols]), the
+    resolution length of each placed column from its row of R = I - S^-1 Gamma over the placed columns of
+    its block. Returns (kept, diagonal, sd, length) or raises ValueError with the reference's reason."""
+    rs = np.asarray(row_start, np.int64)
+    ci = np.asarray(column, np.int64)
+    va = np.asarray(values, np.float64)
+    A = np.zeros((rows, cols))
+    for r in range(rows):
+        A[r, ci[rs[r]:rs[r + 1]]] += va[rs[r]:rs[r + 1]]
+    group = np.full(cols, -1)
+    for g, cs in enumerate(eliminate):
+        group[np.asarray(cs)] = g
+    kept = np.flatnonzero(group < 0)
+    n = kept.size
+    if n > max_unknowns:
+        raise ValueError("too many unknowns")
+    for r in range(rows):
+        gs = set(group[ci[rs[r]:rs[r + 1]]].tolist()) - {-1}
+        if len(gs) > 1:
+            raise ValueError("a row couples the hypocentres of two events")
+        if r >= data_rows and gs and np.any(group[ci[rs[r]:rs[r + 1]]] < 0):
+            raise ValueError("a regularisation row couples model and hypocentre columns")
+    Areg = A[data_rows:][:, kept]
+    gamma = Areg.T @ Areg
+    N = A.T @ A
+    S = N[np.ix_(kept, kept)].copy()
+    for cs in eliminate:
+        cs = np.asarray(cs)
+        live = cs[np.diag(N)[cs] > 0]
+        if live.size == 0:
+            continue
+        Ngg = N[np.ix_(live, live)]
+        try:
+            np.linalg.cholesky(Ngg)
+        except np.linalg.LinAlgError:
+            continue
+        Nkg = N[np.ix_(kept, live)]
+        S -= Nkg @ np.linalg.solve(Ngg, Nkg.T)
+    try:
+        L = np.linalg.cholesky(S)
+    except np.linalg.LinAlgError:
+        raise ValueError("the reduced normal matrix is not positive definite")
+    X = np.linalg.solve(L, np.eye(n))       # L^-1
+    Sinv = X.T @ X
+    sd = np.sqrt(np.diag(Sinv))
+    R = np.eye(n) - Sinv @ gamma
+    diag = np.diag(R).copy()
+    length = np.full(n, np.nan)
+    if places is not None:
+        xyz = np.asarray(places[0], np.float64)[kept]
+        blk = np.asarray(places[1])[kept]
+        for i in range(n):
+            if blk[i] < 0:
+                continue
+            same = blk == blk[i]
+            r2 = R[i, same] ** 2
+            d2 = np.sum((xyz[same] - xyz[i]) ** 2, axis=1)
+            den = r2.sum()
+            length[i] = math.sqrt((r2 * d2).sum() / den) if den > 0 else np.nan
+    return kept, diag, sd, length
