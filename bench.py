@@ -252,6 +252,55 @@ def q_leg(dev, wl, cpu: bool):
     return out
 
 
+def formal_leg(dev, args, cpu: bool):
+    """The exact formal resolution (FormalResolution.Compute) of the Q system of the 61 x 61 x 31 case
+    inverted on every second node (31 x 31 x 16 = 15,376 nodes and 25 station terms: 15,401 unknowns,
+    under the reference's 20,000 limit): S = A^T A, its Cholesky factor, the inverse, the diagonal, the
+    standard deviations and the resolution length of every node. The CPU baseline is LAPACK's Cholesky
+    and triangular inverse (numpy / scipy on the host cores) of the same S: the n^3 part only."""
+    from tomostar_amd import formal as F
+    from tomostar_amd import qtomography as QT
+    from tomostar_amd import workloads as W
+
+    wl = W.one_iteration_case(seed=0)
+    c = QT.q_case(dev, wl, step=2)
+    inv = c.grid
+    rows = np.flatnonzero(c.rays.counts >= 0).astype(np.int32)
+    s = QT.QTomographySettings()
+    d = c.data
+    sysd = QT.build_qsystem(dev, inv, s, c.rays, rows, d.station[rows], d.tstar[rows], d.sigma[rows], d.nstations,
+                            c.slowness, 1 / 250, s.damping, s.smoothing)
+    from tomostar_amd.system import System
+    rs, ci, va = System(sysd.matrix, sysd.rhs).read()
+    m, cols, nd = sysd.matrix.rows, sysd.matrix.cols, sysd.layout["data_rows"]
+    sysd.matrix.release()
+    c.rays.release()
+    places = F.add_places(F.new_places(cols), inv, 0, 0)
+    F.compute(dev, m, cols, rs, ci, va, nd, places=places)  # warm-up: code objects
+    t0 = time.perf_counter()
+    res = F.compute(dev, m, cols, rs, ci, va, nd, places=places)
+    sec = time.perf_counter() - t0
+    n = res.columns.size
+    out = {"workload": f"Q system of 3750 P t* on 31x31x16 nodes + 25 station terms ({n} unknowns), resolution "
+                       "diagonal, standard deviations and resolution lengths",
+           "unknowns": int(n), "gpu_seconds": round(sec, 3),
+           "median_diagonal": round(float(np.nanmedian(res.diagonal[:inv.count])), 4),
+           "median_length_km": round(float(np.nanmedian(res.length_km[:inv.count])), 2)}
+    if cpu:
+        import scipy.linalg as SL
+        import scipy.sparse as SP
+
+        A = SP.csr_matrix((va, ci, rs), shape=(m, cols))
+        S = (A.T @ A).toarray()
+        t0 = time.perf_counter()
+        Lf = np.linalg.cholesky(S)
+        SL.lapack.dtrtri(Lf, lower=1)
+        out["cpu_seconds_cholesky_and_inverse"] = round(time.perf_counter() - t0, 3)
+        out["cpu_sample"] = "LAPACK dpotrf + dtrtri of the same S (numpy / scipy, host cores); lengths not timed"
+    log(f"formal leg: {out}")
+    return out
+
+
 def location_leg(wl, ts, args, cpu: bool):
     """Event location's grid search over the 400 resident tables of the headline solve: events at random
     nodes with 40 picks each (random station x phase tables, origin time, 0.05 s noise, quarter/half/
@@ -611,6 +660,7 @@ def main() -> int:
     ap.add_argument("--uneven-problems", type=int, default=64, help="problems of the uneven-batch leg (0 skips it)")
     ap.add_argument("--serving-requests", type=int, default=16, help="requests of the serving leg (0 skips it)")
     ap.add_argument("--locate-events", type=int, default=2000, help="events of the location leg (0 skips it)")
+    ap.add_argument("--no-formal", action="store_true", help="skip the formal resolution leg")
     ap.add_argument("--serving-workers", type=int, default=4)
     ap.add_argument("--serving-budget-gb", type=float, default=96.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -708,6 +758,7 @@ def main() -> int:
     ts.release()
     it_leg = None if args.no_iteration else iteration_leg(dev, cpu=not args.no_cpu)
     uneven = uneven_leg(dev, args) if args.uneven_problems > 0 else None
+    formal = None if args.no_formal else formal_leg(dev, args, cpu=not args.no_cpu)
 
     out = {
         "metric": "eikonal Gnode-upd/s + LSQR iters/s vs plain FP64 GPU; p50/p99 solve latency",
@@ -744,6 +795,7 @@ def main() -> int:
         "uneven": uneven,
         "serving": serving,
         "location": location,
+        "formal_resolution": formal,
     }
     print(json.dumps(out), flush=True)
     dev.close()

@@ -380,8 +380,9 @@ class QCase:
 
     rays: RayBatch
     data: QData
-    slowness: np.ndarray
-    q_true: np.ndarray   # Q on the nodes
+    slowness: np.ndarray  # on the inversion grid
+    q_true: np.ndarray    # Q on the inversion grid's nodes
+    grid: GridDefinition | None = None  # the inversion grid
 
 
 def q_model(grid: GridDefinition, q_mean: float = 250.0) -> np.ndarray:
@@ -396,8 +397,9 @@ def q_model(grid: GridDefinition, q_mean: float = 250.0) -> np.ndarray:
 
 
 def q_case(device: Device, wl, sigma: float = 0.004, seed: int = 0, q_mean: float = 250.0,
-           eik_mode: str = "fast_fp64") -> QCase:
-    """t* = Predict(1/Q) + N(0, sigma) of the P rays of every event x station pair of ``wl``."""
+           eik_mode: str = "fast_fp64", step: int = 1) -> QCase:
+    """t* = Predict(1/Q) + N(0, sigma) of the P rays of every event x station pair of ``wl``; step > 1
+    inverts on every step-th node of the forward grid along each axis (the slowness taken there)."""
     from . import eikonal as E
     g = wl.grid
     sP = wl.slowness_p
@@ -409,12 +411,19 @@ def q_case(device: Device, wl, sigma: float = 0.004, seed: int = 0, q_mean: floa
     ns, ne = len(wl.stations), len(wl.events)
     station = np.tile(np.arange(ns, dtype=np.int32), ne)
     event = np.repeat(np.arange(ne), ns)
-    rays = RayBatch(ts, g, wl.stations, station, wl.events[event])
+    inv, s_inv = g, sP
+    if step > 1:
+        if any((m - 1) % step for m in (g.nx, g.ny, g.nz)):
+            raise ValueError("the step must divide every axis' intervals")
+        inv = GridDefinition(g.min_lon, g.max_lon, g.min_lat, g.max_lat, g.min_depth, g.max_depth,
+                             (g.nx - 1) // step + 1, (g.ny - 1) // step + 1, (g.nz - 1) // step + 1)
+        s_inv = np.ascontiguousarray(sP.reshape(g.shape)[::step, ::step, ::step]).ravel()
+    rays = RayBatch(ts, inv, wl.stations, station, wl.events[event])
     ts.release()
-    Qn = q_model(g, q_mean)
+    Qn = q_model(inv, q_mean)
     rows = np.flatnonzero(rays.counts >= 0).astype(np.int32)
-    pred, _ = q_predict(rays, rows, sP, 1 / Qn, 1.0)
+    pred, _ = q_predict(rays, rows, s_inv, 1 / Qn, 1.0)
     tstar = np.full(station.size, np.nan)
     rng = np.random.default_rng(seed)
     tstar[rows] = pred + sigma * rng.standard_normal(rows.size)
-    return QCase(rays, QData(station, tstar, np.full(station.size, sigma), ns), sP, Qn)
+    return QCase(rays, QData(station, tstar, np.full(station.size, sigma), ns), s_inv, Qn, inv)
