@@ -34,17 +34,26 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kT = 64;   // tile and panel edge
 constexpr int kKC = 32;  // K chunk staged in LDS per step
 
+// The K range of output tile (ti, tj) when operands are triangular: from max(kb_i ti, kb_j tj) tiles
+// (A's row tile zero to the left of its diagonal: kb_i; op(B)'s column tile zero above its diagonal:
+// kb_j) to (ti + 1) tiles when ke_i (A lower triangular), else to K.
+struct KRange {
+    int kb_i, kb_j, ke_i;
+};
+
 // C[M x N] = alpha A[M x K] op(B) + beta C, row-major with leading dimensions; op(B) is B[K x N] or,
 // with kTransB, the transpose of B[N x K]. M, N, K are multiples of 64; one workgroup per 64 x 64 tile
 // of C (grid x: column tiles, y: row tiles); kLower skips the tiles above the diagonal. beta = 0 does
 // not read C.
 template <bool kTransB, bool kLower>
-__global__ __launch_bounds__(256) void fr_gemm(int K, double alpha, const double *__restrict__ A, int64_t lda,
-                                               const double *__restrict__ B, int64_t ldb, double beta,
+__global__ __launch_bounds__(256) void fr_gemm(int K, KRange kr, double alpha, const double *__restrict__ A,
+                                               int64_t lda, const double *__restrict__ B, int64_t ldb, double beta,
                                                double *__restrict__ C, int64_t ldc)
 {
     const int ti = blockIdx.y, tj = blockIdx.x;
     if (kLower && tj > ti) return;
+    const int kbeg = max(kr.kb_i * ti, kr.kb_j * tj) * kT;
+    const int kend = kr.ke_i ? min(K, (ti + 1) * kT) : K;
     __shared__ double As[kT][kKC + 1];
     __shared__ double Bs[kKC][kT + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -54,7 +63,7 @@ __global__ __launch_bounds__(256) void fr_gemm(int K, double alpha, const double
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b] = v4d{0, 0, 0, 0};
     const double *Ab = A + (int64_t)ti * kT * lda;
-    for (int k0 = 0; k0 < K; k0 += kKC) {
+    for (int k0 = kbeg; k0 < kend; k0 += kKC) {
         for (int e = tid; e < kT * kKC; e += 256) {
             const int r = e / kKC, c = e % kKC;
             As[r][c] = Ab[(int64_t)r * lda + k0 + c];
@@ -149,7 +158,9 @@ __global__ void fr_pad_identity(double *S, int64_t ld, int n, int npad)
 }
 
 // Cholesky of the 64 x 64 diagonal block at (k0, k0) in LDS, lower triangle; zeros above the diagonal;
-// bad = 1 when a pivot is not positive.
+// bad = 1 when a pivot is not positive. Step j updates the trailing block with column j and its pivot
+// d (L_il -= L_ij (L_lj / d)) and scales column j - 1, which step j no longer reads: one barrier per
+// column.
 __global__ __launch_bounds__(256) void fr_potrf_diag(double *S, int64_t ld, int k0, int32_t *bad)
 {
     __shared__ double L[kT][kT + 1];
@@ -160,21 +171,24 @@ __global__ __launch_bounds__(256) void fr_potrf_diag(double *S, int64_t ld, int 
         L[r][c] = c <= r ? base[(int64_t)r * ld + c] : 0.0;
     }
     __syncthreads();
+    double dprev = 0;
     for (int j = 0; j < kT; j++) {
-        if (tid == 0) {
-            const double d = L[j][j];
-            if (!(d > 0)) *bad = 1;
-            L[j][j] = sqrt(d);
+        const double d = L[j][j];
+        if (tid == 0 && !(d > 0)) *bad = 1;
+        if (j > 0) {
+            const double sq = sqrt(dprev);
+            for (int i = j - 1 + tid; i < kT; i += 256) L[i][j - 1] = i == j - 1 ? sq : L[i][j - 1] / sq;
         }
-        __syncthreads();
-        for (int i = j + 1 + tid; i < kT; i += 256) L[i][j] /= L[j][j];
-        __syncthreads();
+        const double rd = 1.0 / d;
         for (int e = tid; e < kT * kT; e += 256) {
             const int i = e / kT, l = e % kT;
-            if (l > j && l <= i) L[i][l] -= L[i][j] * L[l][j];
+            if (l > j && l <= i) L[i][l] -= L[i][j] * (L[l][j] * rd);
         }
+        dprev = d;
         __syncthreads();
     }
+    if (tid == 0) L[kT - 1][kT - 1] = sqrt(dprev);
+    __syncthreads();
     for (int e = tid; e < kT * kT; e += 256) {
         const int r = e / kT, c = e % kT;
         base[(int64_t)r * ld + c] = L[r][c];
@@ -241,6 +255,24 @@ __global__ __launch_bounds__(256) void fr_transpose(const double *__restrict__ X
     for (int e = tid; e < kT * kT; e += 256) {
         const int r = e / kT, c = e % kT;
         Y[((int64_t)bj * kT + r) * ld + (int64_t)bi * kT + c] = t[c][r];
+    }
+}
+
+// The tiles above the diagonal from the transposed tiles below it (a symmetric matrix of which the
+// lower tiles are computed).
+__global__ __launch_bounds__(256) void fr_mirror(double *S, int64_t ld)
+{
+    __shared__ double t[kT][kT + 1];
+    const int bi = blockIdx.y, bj = blockIdx.x, tid = threadIdx.x;
+    if (bj <= bi) return;
+    for (int e = tid; e < kT * kT; e += 256) {
+        const int r = e / kT, c = e % kT;
+        t[r][c] = S[((int64_t)bj * kT + r) * ld + (int64_t)bi * kT + c];
+    }
+    __syncthreads();
+    for (int e = tid; e < kT * kT; e += 256) {
+        const int r = e / kT, c = e % kT;
+        S[((int64_t)bi * kT + r) * ld + (int64_t)bj * kT + c] = t[c][r];
     }
 }
 
@@ -320,10 +352,10 @@ unsigned nblocks(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, (n 
 
 template <bool kTransB, bool kLower>
 void gemm(hipStream_t st, int M, int N, int K, double alpha, const double *A, int64_t lda, const double *B,
-          int64_t ldb, double beta, double *C, int64_t ldc)
+          int64_t ldb, double beta, double *C, int64_t ldc, KRange kr = {0, 0, 0})
 {
     if (M <= 0 || N <= 0) return;
-    fr_gemm<kTransB, kLower><<<dim3(N / kT, M / kT), 256, 0, st>>>(K, alpha, A, lda, B, ldb, beta, C, ldc);
+    fr_gemm<kTransB, kLower><<<dim3(N / kT, M / kT), 256, 0, st>>>(K, kr, alpha, A, lda, B, ldb, beta, C, ldc);
     TS_HIP(hipGetLastError());
 }
 
@@ -336,8 +368,11 @@ void trtri(hipStream_t st, double *S, int64_t ld, int off, int size, double *T)
     trtri(st, S, ld, off, h, T);
     trtri(st, S, ld, off + h, size - h, T);
     double *L21 = S + (int64_t)(off + h) * ld + off;
-    gemm<false, false>(st, size - h, h, h, 1.0, L21, ld, S + (int64_t)off * ld + off, ld, 0.0, T, h);
-    gemm<false, false>(st, size - h, h, size - h, -1.0, S + (int64_t)(off + h) * ld + off + h, ld, T, h, 0.0, L21, ld);
+    // X11 is lower triangular: column tile tj of L21 X11 sums from tile tj; X22 too: row tile ti of
+    // X22 T sums up to tile ti
+    gemm<false, false>(st, size - h, h, h, 1.0, L21, ld, S + (int64_t)off * ld + off, ld, 0.0, T, h, {0, 1, 0});
+    gemm<false, false>(st, size - h, h, size - h, -1.0, S + (int64_t)(off + h) * ld + off + h, ld, T, h, 0.0, L21,
+                       ld, {0, 0, 1});
 }
 
 // Counting-sort transpose with rows in increasing order within each column (SparseMatrix.cs:79-97).
@@ -631,8 +666,11 @@ int ts_formal_resolution(ts_context *ctx, int rows, int cols, const int32_t *row
         fr_diag_sd<<<n, 256, 0, st>>>(n, Y.p, ld, d_grs.p, d_gci.p, d_gva.p, d_diag.p, d_sd.p);
         TS_HIP(hipGetLastError());
         if (places) {
-            // S^-1 = Y Y^T into the first array
-            gemm<true, false>(st, npad, npad, npad, 1.0, Y.p, ld, Y.p, ld, 0.0, S.p, ld);
+            // S^-1 = Y Y^T into the first array: Y is upper triangular, so tile (ti, tj <= ti) sums from
+            // tile ti; the upper tiles are the mirror of the lower ones
+            gemm<true, true>(st, npad, npad, npad, 1.0, Y.p, ld, Y.p, ld, 0.0, S.p, ld, {1, 1, 0});
+            fr_mirror<<<dim3(npad / kT, npad / kT), 256, 0, st>>>(S.p, ld);
+            TS_HIP(hipGetLastError());
             std::vector<double> pos(3 * (size_t)n);
             std::vector<int32_t> blk(n);
             for (int i = 0; i < n; i++) {
