@@ -164,3 +164,27 @@ def test_velocity_system_of_a_small_grid(device, oracle):
     assert np.array_equal(maps.resolution, res.diagonal[:g.count])
     sysd.matrix.release()
     batch.release()
+
+
+def test_edges(device, oracle):
+    """Every column separated out (nothing kept), no columns at all, a single kept column, and a kept
+    count of exactly one tile."""
+    from tomostar_amd import formal as F
+
+    A = np.array([[1.0, 2.0], [0.5, 0.0], [0.0, 0.5]])
+    rs, ci, va = _csr(A)
+    res = F.compute(device, 3, 2, rs, ci, va, 1, [[0, 1]])
+    assert res.columns.size == 0
+    res = F.compute(device, 0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0), 0)
+    assert res.columns.size == 0
+    A = np.array([[2.0], [0.5]])
+    rs, ci, va = _csr(A)
+    res = F.compute(device, 2, 1, rs, ci, va, 1)
+    assert np.isclose(res.diagonal[0], 4.0 / 4.25, rtol=1e-14) and np.isclose(res.standard_deviation[0],
+                                                                              (1 / 4.25) ** 0.5, rtol=1e-14)
+    rng = np.random.default_rng(9)
+    G = rng.standard_normal((100, 64)) * (rng.random((100, 64)) < 0.2)
+    A = np.vstack([G, 0.5 * np.eye(64)])
+    rs, ci, va = _csr(A)
+    _compare(F.compute(device, A.shape[0], 64, rs, ci, va, 100),
+             oracle.formal_resolution(A.shape[0], 64, rs, ci, va, 100))

@@ -155,3 +155,52 @@ def test_trade_off_points(device, qcase):
         if np.all((p.q > 1.001 * s.q_min) & (p.q < 0.999 * s.q_max)):
             x = 1 / (p.q * qt.q0) - 1
             assert np.isclose(p.roughness, T.roughness(wl.grid, x, 0.1), rtol=1e-9)
+
+
+def test_edges_and_invalid_arguments(device, oracle, qcase):
+    """No rows (the regularisation alone, unit scales); no station terms with first differences, against
+    the oracle; q_old without m_old, a bad station and a non-positive sigma are refused."""
+    from tomostar_amd import _lib
+    from tomostar_amd import qtomography as QT
+    from tomostar_amd.system import System
+
+    wl, c = qcase
+    g = wl.grid
+    og = oracle.griddef(*dataclasses.astuple(g))
+    d = c.data
+    s = QT.QTomographySettings()
+    pred, tt = QT.q_predict(c.rays, np.zeros(0, np.int32), c.slowness, None, 1 / 300)
+    assert pred.size == 0 and tt.size == 0
+    sysd = QT.build_qsystem(device, g, s, c.rays, np.zeros(0, np.int32), [], [], [], d.nstations, c.slowness,
+                            1 / 300, s.damping, s.smoothing)
+    assert sysd.scales == (1.0, 1.0) and sysd.layout["data_rows"] == 0
+    assert sysd.matrix.rows == 2 * g.count + d.nstations + 1
+    sysd.matrix.release()
+    rows = np.flatnonzero(c.rays.counts >= 0).astype(np.int32)[:500]
+    s2 = QT.QTomographySettings(station_terms=False, smoothing_method="gradient", smoothing_scale="nodes")
+    sysd = QT.build_qsystem(device, g, s2, c.rays, rows, d.station[rows], d.tstar[rows], d.sigma[rows], d.nstations,
+                            c.slowness, 1 / 300, 1.5, 4.0)
+    rs, ci, va = System(sysd.matrix, sysd.rhs).read()
+    hrs, hnd, hln = _host_rows(c.rays, rows)
+    opts = dict(station_terms=0, gradient_smoothing=1, smoothing_in_nodes=1, nstations=d.nstations, damping=1.5,
+                smoothing=4.0, vertical_weight=s2.vertical_weight, q0=1 / 300)
+    ors, oci, ova, orhs, dims, oscales = oracle.build_qsystem(og, opts, hrs, hnd, hln, d.station[rows], d.tstar[rows],
+                                                              d.sigma[rows], c.slowness)
+    assert sysd.layout["off_station"] == -1 and sysd.matrix.cols == g.count
+    assert np.array_equal(rs, ors) and np.array_equal(ci, oci)
+    assert np.allclose(va, ova, rtol=1e-13, atol=0) and np.allclose(sysd.rhs, orhs, rtol=1e-13, atol=0)
+    sysd.matrix.release()
+    with pytest.raises(_lib.HipError) as e:
+        QT.build_qsystem(device, g, s, c.rays, rows, d.station[rows], d.tstar[rows], d.sigma[rows], d.nstations,
+                         c.slowness, 1 / 300, 1.0, 1.0, q_old=np.full(g.count, 1 / 300))
+    assert e.value.status == _lib.TS_EINVAL
+    bad = d.station[rows].copy()
+    bad[3] = d.nstations
+    with pytest.raises(_lib.HipError, match="station"):
+        QT.build_qsystem(device, g, s, c.rays, rows, bad, d.tstar[rows], d.sigma[rows], d.nstations, c.slowness,
+                         1 / 300, 1.0, 1.0)
+    sig = d.sigma[rows].copy()
+    sig[0] = 0.0
+    with pytest.raises(_lib.HipError, match="sigma"):
+        QT.build_qsystem(device, g, s, c.rays, rows, d.station[rows], d.tstar[rows], sig, d.nstations, c.slowness,
+                         1 / 300, 1.0, 1.0)
