@@ -162,6 +162,10 @@ def test_velocity_system_of_a_small_grid(device, oracle):
     assert np.all(res.diagonal < 1 + 1e-12) and np.all(res.standard_deviation > 0)
     maps = F.on_nodes(res, cols, g.count, lay["offP"])
     assert np.array_equal(maps.resolution, res.diagonal[:g.count])
+    # TravelTimeTomography.Formal's wrapper: the same groups and places, the blocks mapped to the nodes
+    pm, sm, fr = F.velocity_formal(device, sysd, g, 40)
+    assert np.array_equal(fr.diagonal, res.diagonal) and np.array_equal(pm.resolution, maps.resolution)
+    assert np.array_equal(sm.sigma, F.on_nodes(res, cols, g.count, lay["offS"]).sigma)
     sysd.matrix.release()
     batch.release()
 
@@ -188,3 +192,31 @@ def test_edges(device, oracle):
     rs, ci, va = _csr(A)
     _compare(F.compute(device, A.shape[0], 64, rs, ci, va, 100),
              oracle.formal_resolution(A.shape[0], 64, rs, ci, va, 100))
+
+
+def test_q_run_with_formal_resolution(device, oracle):
+    """QTomography.Run with the formal resolution on (QTomography.cs:480-497): the 61 x 61 x 31 case's P
+    t* inverted on every third node (21 x 21 x 11 nodes and 25 station terms), the last pass's system
+    against the oracle; too small a limit is logged, not raised."""
+    from tomostar_amd import formal as F
+    from tomostar_amd import qtomography as QT
+    from tomostar_amd import workloads as W
+
+    wl = W.one_iteration_case(seed=0)
+    c = QT.q_case(device, wl, step=3)
+    s = QT.QTomographySettings(lsqr_iterations=60, nonlinear_iterations=2, formal_resolution=True)
+    qt = QT.QTomography(device, c.grid, s)
+    res = qt.run(c.rays, c.data, c.slowness)
+    assert res.formal is not None and res.formal_reason == ""
+    (rs, ci, va), rows, cols, nd = qt.last_system
+    places = F.add_places(F.new_places(cols), c.grid, 0, 0)
+    kept, diag, sd, length = oracle.formal_resolution(rows, cols, rs, ci, va, nd, (), 20000, places)
+    n = c.grid.count
+    assert np.allclose(res.formal.resolution, diag[:n], rtol=1e-9, atol=1e-12)
+    assert np.allclose(res.formal.sigma, sd[:n], rtol=1e-9)
+    assert np.allclose(res.formal.length_km, length[:n], rtol=1e-7, atol=1e-9)
+    s2 = QT.QTomographySettings(lsqr_iterations=20, nonlinear_iterations=1, formal_resolution=True,
+                                formal_max_unknowns=1000)
+    res2 = QT.QTomography(device, c.grid, s2).run(c.rays, c.data, c.slowness)
+    assert res2.formal is None and "above the limit" in res2.formal_reason
+    c.rays.release()

@@ -129,3 +129,24 @@ def compute(device: Device, rows: int, cols: int, row_start, column, values, dat
         raise _lib.HipError(rc, r.value.decode(errors="replace"))
     n = nk.value
     return FormalResolutionResult(kept[:n].copy(), diag[:n].copy(), sd[:n].copy(), ln[:n].copy(), secs.value)
+
+
+def velocity_formal(device: Device, system, grid: GridDefinition, nevents: int, max_unknowns: int = 20000):
+    """TravelTimeTomography.Formal (Tomography/TravelTimeTomography.cs:525-548) on a plain node grid: the
+    system of the last iteration (``system.build_system``'s ``System``, FP64 values kept), the 4
+    hypocentre columns of every event separated out, the P and S blocks placed (blocks 0 and 1), the
+    results mapped to the nodes. Returns (P maps or None, S maps or None, the result)."""
+    lay = system.layout
+    rs, ci, va = system.read()
+    cols, rows = system.matrix.cols, system.matrix.rows
+    groups = [] if lay["offH"] < 0 else [list(range(lay["offH"] + 4 * e, lay["offH"] + 4 * e + 4))
+                                         for e in range(nevents)]
+    places = new_places(cols)
+    if lay["offP"] >= 0:
+        add_places(places, grid, lay["offP"], 0)
+    if lay["offS"] >= 0:
+        add_places(places, grid, lay["offS"], 1)
+    fr = compute(device, rows, cols, rs, ci, va, lay["data_rows"], groups, max_unknowns, places)
+    p = on_nodes(fr, cols, grid.count, lay["offP"]) if lay["offP"] >= 0 else None
+    s = on_nodes(fr, cols, grid.count, lay["offS"]) if lay["offS"] >= 0 else None
+    return p, s, fr

@@ -71,6 +71,8 @@ class QTomographySettings:
     q_min: float = 10.0
     q_max: float = 5000.0
     outlier_mads: float = 4.0
+    formal_resolution: bool = False    # FormalResolutionSettings.Enabled, on the last system
+    formal_max_unknowns: int = 20000   # FormalResolutionSettings.MaxUnknowns
 
 
 @dataclass
@@ -123,6 +125,8 @@ class QTomographyResult:
     gauss_newton: tuple      # (iterations, RMS change of ln q at the last)
     residual: np.ndarray     # per arrival (NaN: no ray)
     seconds: dict
+    formal: object = None    # FormalResolutionMaps of the unknowns on the nodes, when asked for and possible
+    formal_reason: str = ""  # why it is missing when asked for
 
 
 @dataclass
@@ -295,6 +299,10 @@ class QTomography:
         self.seconds["predict"] += t3 - t2
         p = QPass(x, q, terms, res[0].iterations, res[0].stop_code, sysd.matrix.rows, sysd.matrix.cols,
                   sysd.matrix.nnz)
+        if s.formal_resolution:  # the last pass's system, for the formal resolution (:480-497)
+            from .system import System
+            self.last_system = (System(sysd.matrix, sysd.rhs).read(), sysd.matrix.rows, sysd.matrix.cols,
+                                sysd.layout["data_rows"])
         sysd.matrix.release()
         return p, m_nodes
 
@@ -348,8 +356,25 @@ class QTomography:
         rejected = self.reject_outliers()
         if rejected > 0:
             p = self.solve(self.s.damping, self.s.smoothing)
-        return QTomographyResult(1 / p.q, self._dws(), self.rms_before, self.rms(), 1 / self.q0, p.terms, rejected,
-                                 p.iterations, p.stop_code, self.gauss_newton, self.residual.copy(), dict(self.seconds))
+        out = QTomographyResult(1 / p.q, self._dws(), self.rms_before, self.rms(), 1 / self.q0, p.terms, rejected,
+                                p.iterations, p.stop_code, self.gauss_newton, self.residual.copy(), dict(self.seconds))
+        if self.s.formal_resolution:
+            self._formal(out)
+        return out
+
+    def _formal(self, out: QTomographyResult) -> None:
+        """The formal resolution of the last system (:480-497): no groups to separate, the node block
+        placed; the reference's declines are logged, not raised."""
+        from . import formal as F
+        (rs, ci, va), rows, cols, data_rows = self.last_system
+        places = F.add_places(F.new_places(cols), self.grid, 0, 0)
+        try:
+            fr = F.compute(self.device, rows, cols, rs, ci, va, data_rows, (), self.s.formal_max_unknowns, places)
+        except _lib.HipError as e:
+            out.formal_reason = str(e)
+            return
+        out.formal = F.on_nodes(fr, cols, self.grid.count, 0)
+        out.seconds["formal"] = fr.seconds
 
     # ---- TradeOff (:520-554): every pair a complete solve on the same data
     def trade_off(self, rays: RayBatch, data: QData, slowness, pairs) -> list[QTradeOffPoint]:
