@@ -30,13 +30,14 @@ HBM_PEAK_GBS = 8000.0
 # are counted as served from cache): FP32 times 4+4+4, FP64 times 8+8+4
 BYTES_PER_UPDATE_FP32 = 12.0
 BYTES_PER_UPDATE_FP64 = 20.0
-# HBM bytes per launch of the sweep kernels on this workload: FETCH_SIZE + WRITE_SIZE from rocprofv3
-# --pmc (KiB per dispatch, averaged over the solve's 24,048 launches of each kernel), scaled by what the
-# same counters read for an elementwise copy of 1 GiB (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000). The
-# runs and the figures are summarised in profiles/eik_fast_plane_batched_400x201x201x101.md.
+# HBM bytes per launch (one plane step: the three chunks' kernels of one ordering's plane) of the sweep
+# kernels on this workload: FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc (KiB per dispatch, summed over
+# the solve's dispatches of each width and divided by its plane steps), scaled by what the same
+# counters read for an elementwise copy of 1 GiB (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000). The runs and
+# the figures are summarised in profiles/eik_fast_plane_batched_400x201x201x101.md.
 TRAFFIC_BYTES_PER_LAUNCH: dict = {
-    "eik_fast_plane<double>": 9.383e7,
-    "eik_fast_plane<float>": 3.056e7,
+    "eik_fast_plane<double>": 9.475e7,
+    "eik_fast_plane<float>": 3.516e7,
 }
 
 
@@ -649,7 +650,9 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    # two warm-up solves: the first counts node updates (its own graphs), the second captures the graphs
+    # the timed solves launch
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--stations", type=int, default=200)
     ap.add_argument("--plain-fields", type=int, default=23, help="fields of the plain-FP64 sample (one batch); 0 skips it")
     ap.add_argument("--no-cpu", action="store_true")
@@ -714,7 +717,8 @@ def main() -> int:
     upd_rate = evals / (total / args.steps) / 1e9
     # the roofline of the dominant kernel: per launch, algorithmic bytes / average launch duration
     # (HIP events around the rounds on the solve's stream; a round is the plane launches plus two
-    # small bookkeeping kernels, whose time is charged to the plane kernel here)
+    # small bookkeeping kernels, whose time is charged to the plane kernel here). A launch is one plane
+    # step: with the groups split over streams, the chunks' kernels of a step run side by side.
     launches32 = st["rounds_fp32"] * st["plane_launches_per_round"]
     launches64 = st["rounds_fp64"] * st["plane_launches_per_round"]
     kern = {
