@@ -42,6 +42,14 @@ struct MgSeg {
     int problem, list_off, count, block_start;
 };
 
+// A chunk of the problems sweeping on its own stream: its segment tables and, per width, its graph of
+// plane launches.
+struct MgChunk {
+    std::vector<int> seg_off, seg_blocks;
+    dbuf<MgSeg> segs;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
 struct ts_gridbatch {
     ts_context *ctx = nullptr;
     int nprob = 0;
@@ -66,7 +74,23 @@ struct ts_gridbatch {
     std::vector<int> tuned;               // nodes per thread chosen per size class
     double seconds = 0, sweep_seconds = 0;
     int last_mode = -1;
+    static constexpr int kMaxChunks = 4;
+    MgChunk chunk[kMaxChunks];
+    int nchunks = 0;                      // > 1: the problems sweep in chunks side by side
+    hipStream_t cstream[kMaxChunks] = {};
+    hipEvent_t cfork = nullptr, cjoin[kMaxChunks] = {};
     std::mutex gate;
+    ~ts_gridbatch()
+    {
+        for (auto &c : chunk)
+            for (auto &g : c.graph)
+                if (g) (void)hipGraphExecDestroy(g);
+        for (auto &sx : cstream)
+            if (sx) (void)hipStreamDestroy(sx);
+        for (auto &e : cjoin)
+            if (e) (void)hipEventDestroy(e);
+        if (cfork) (void)hipEventDestroy(cfork);
+    }
 };
 
 namespace ts {
@@ -248,15 +272,16 @@ unsigned mg_blocks(int64_t n, int bs) { return (unsigned)std::max<int64_t>(1, (n
 int size_class(int64_t nodes) { return nodes < 200000 ? 0 : nodes < 800000 ? 1 : 2; }
 
 // The segment tables of every plane index for the problems in `which` (all when empty).
-void build_segments(ts_gridbatch &b, const std::vector<int> &which)
+void build_segments_into(ts_gridbatch &b, const std::vector<int> &which, std::vector<int> &seg_off,
+                         std::vector<int> &seg_blocks, dbuf<MgSeg> &segs)
 {
     std::vector<MgSeg> all;
-    b.seg_off.assign(b.maxP + 1, 0);
-    b.seg_blocks.assign(b.maxP, 0);
+    seg_off.assign(b.maxP + 1, 0);
+    seg_blocks.assign(b.maxP, 0);
     std::vector<char> on(b.nprob, which.empty() ? 1 : 0);
     for (int q : which) on[q] = 1;
     for (int p = 0; p < b.maxP; p++) {
-        b.seg_off[p] = (int)all.size();
+        seg_off[p] = (int)all.size();
         int blocks = 0;
         for (int q = 0; q < b.nprob; q++) {
             const auto &po = b.plane_off[q];
@@ -268,11 +293,53 @@ void build_segments(ts_gridbatch &b, const std::vector<int> &which)
             all.push_back(MgSeg{q, (int)(b.list_base[q] + po[p]), cnt, blocks});
             blocks += nb;
         }
-        b.seg_blocks[p] = blocks;
+        seg_blocks[p] = blocks;
     }
-    b.seg_off[b.maxP] = (int)all.size();
-    b.segs.alloc(std::max<size_t>(1, all.size()));
-    if (!all.empty()) TS_HIP(hipMemcpy(b.segs.p, all.data(), all.size() * sizeof(MgSeg), hipMemcpyHostToDevice));
+    seg_off[b.maxP] = (int)all.size();
+    segs.alloc(std::max<size_t>(1, all.size()));
+    if (!all.empty()) TS_HIP(hipMemcpy(segs.p, all.data(), all.size() * sizeof(MgSeg), hipMemcpyHostToDevice));
+}
+
+// Chunks sweeping side by side when rounds can run as graphs (TOMOSTAR_GRIDBATCH_STREAMS overrides;
+// 0 or 1: every problem in every launch).
+int gridbatch_streams(const ts_gridbatch &b)
+{
+    int n = b.ctx->use_graphs ? 3 : 0;
+    if (const char *e = getenv("TOMOSTAR_GRIDBATCH_STREAMS")) n = atoi(e);
+    return std::max(0, std::min({n, b.nprob, (int)ts_gridbatch::kMaxChunks}));
+}
+
+void build_segments(ts_gridbatch &b, const std::vector<int> &which)
+{
+    build_segments_into(b, which, b.seg_off, b.seg_blocks, b.segs);
+    // the chunks: every problem, largest first to the chunk with the fewest nodes so far
+    for (auto &c : b.chunk)
+        for (auto &g : c.graph)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    b.nchunks = 0;
+    const int S = which.empty() ? gridbatch_streams(b) : 0;
+    if (S < 2) return;
+    std::vector<int> order(b.nprob);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return b.base[x + 1] - b.base[x] > b.base[y + 1] - b.base[y]; });
+    std::vector<std::vector<int>> part(S);
+    std::vector<int64_t> load(S, 0);
+    for (int q : order) {
+        const int c = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        part[c].push_back(q);
+        load[c] += b.base[q + 1] - b.base[q];
+    }
+    for (int c = 0; c < S; c++) {
+        std::sort(part[c].begin(), part[c].end());
+        build_segments_into(b, part[c], b.chunk[c].seg_off, b.chunk[c].seg_blocks, b.chunk[c].segs);
+        if (c > 0) {
+            if (!b.cstream[c]) TS_HIP(hipStreamCreateWithFlags(&b.cstream[c], hipStreamNonBlocking));
+            if (!b.cjoin[c]) TS_HIP(hipEventCreateWithFlags(&b.cjoin[c], hipEventDisableTiming));
+        }
+    }
+    if (!b.cfork) TS_HIP(hipEventCreateWithFlags(&b.cfork, hipEventDisableTiming));
+    b.nchunks = S;
 }
 
 template <typename R>
@@ -291,17 +358,65 @@ MgArgs<R> mg_args(ts_gridbatch &b)
 }
 
 template <typename R>
-void mg_round(ts_gridbatch &b, hipStream_t st, int hold)
+void mg_planes(ts_gridbatch &b, hipStream_t st, const std::vector<int> &seg_off, const std::vector<int> &seg_blocks,
+               const MgSeg *segs)
 {
     const MgArgs<R> A = mg_args<R>(b);
     for (int dir = 0; dir < 8; dir++)
         for (int p = 0; p < b.maxP; p++) {
-            const int ns = b.seg_off[p + 1] - b.seg_off[p];
-            if (ns == 0 || b.seg_blocks[p] == 0) continue;
-            mg_plane<R><<<b.seg_blocks[p], kMgBlock, 0, st>>>(A, b.segs.p + b.seg_off[p], ns, dir);
+            const int ns = seg_off[p + 1] - seg_off[p];
+            if (ns == 0 || seg_blocks[p] == 0) continue;
+            mg_plane<R><<<seg_blocks[p], kMgBlock, 0, st>>>(A, segs + seg_off[p], ns, dir);
         }
+}
+
+void mg_end(ts_gridbatch &b, hipStream_t st, int hold)
+{
     mg_round_end<<<mg_blocks(b.nprob, 64), 64, 0, st>>>(b.nprob, b.phase.p, b.changed.p, b.rounds.p, b.cap.p,
                                                        b.status.p, b.reset.p, hold);
+}
+
+template <typename R>
+void mg_round(ts_gridbatch &b, hipStream_t st, int hold)
+{
+    mg_planes<R>(b, st, b.seg_off, b.seg_blocks, b.segs.p);
+    mg_end(b, st, hold);
+}
+
+// Chunk c's plane launches as a graph captured on stream cs (the arrays they read stay where they are
+// for the batch's life; the segment tables are rebuilt only with the graphs).
+template <typename R>
+hipGraphExec_t mg_chunk_graph(ts_gridbatch &b, int c, hipStream_t cs)
+{
+    hipGraphExec_t &ge = b.chunk[c].graph[sizeof(R) == 8];
+    if (ge) return ge;
+    hipGraph_t gr;
+    TS_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    mg_planes<R>(b, cs, b.chunk[c].seg_off, b.chunk[c].seg_blocks, b.chunk[c].segs.p);
+    TS_HIP(hipStreamEndCapture(cs, &gr));
+    hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    TS_HIP(e);
+    return ge;
+}
+
+// A round of the chunks side by side: chunk 0 on the batch's stream, the others on their own streams
+// between a fork and a join.
+template <typename R>
+void mg_round_chunks(ts_gridbatch &b, hipStream_t st, int hold)
+{
+    const int S = b.nchunks;
+    std::vector<hipGraphExec_t> g(S);
+    for (int c = 0; c < S; c++) g[c] = mg_chunk_graph<R>(b, c, c == 0 ? st : b.cstream[c]);
+    TS_HIP(hipEventRecord(b.cfork, st));
+    for (int c = 1; c < S; c++) {
+        TS_HIP(hipStreamWaitEvent(b.cstream[c], b.cfork, 0));
+        TS_HIP(hipGraphLaunch(g[c], b.cstream[c]));
+        TS_HIP(hipEventRecord(b.cjoin[c], b.cstream[c]));
+    }
+    TS_HIP(hipGraphLaunch(g[0], st));
+    for (int c = 1; c < S; c++) TS_HIP(hipStreamWaitEvent(st, b.cjoin[c], 0));
+    mg_end(b, st, hold);
 }
 
 void mg_start(ts_gridbatch &b, bool fp32, hipStream_t st)
@@ -346,7 +461,8 @@ void mg_rounds(ts_gridbatch &b, const volatile int32_t *cancel, int hold, int ma
         if (cancel && *cancel) fail(TS_CANCELLED, "cancelled");
         TS_HIP(hipMemsetAsync(b.status.p, 0, 8, st));
         TS_HIP(hipEventRecord(e0, st));
-        mg_round<R>(b, st, hold);
+        if (b.nchunks > 1) mg_round_chunks<R>(b, st, hold);
+        else mg_round<R>(b, st, hold);
         TS_HIP(hipEventRecord(e1, st));
         if (!hold) mg_reset_dirty<<<mg_blocks(total, 256), 256, 0, st>>>(b.dirty.p, d_base, b.nprob, b.reset.p, total);
         TS_HIP(hipMemcpyAsync(h_status, b.status.p, 8, hipMemcpyDeviceToHost, st));
