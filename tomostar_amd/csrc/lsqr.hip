@@ -334,6 +334,74 @@ __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__rest
     if (grp == 0) partial[(size_t)blockIdx.x * KP + j] = red[j];
 }
 
+// lsqr_spmm for KP < 8: 8 lanes per row, 8 / KP of them per right-hand side, each taking every
+// (8 / KP)-th entry of the row; the lanes' sums meet in a fixed shuffle tree. A row of a few hundred
+// entries (a ray among regularisation rows) then takes a few dozen dependent steps instead of holding
+// its wave for the whole row. Same outputs and partials as lsqr_spmm, in another summation order.
+template <int KP, typename T>
+__global__ __launch_bounds__(256) void lsqr_spmm_split(int rows, const int32_t *__restrict__ rs,
+                                                       const int32_t *__restrict__ ci, const float *__restrict__ va,
+                                                       const T *__restrict__ X, T *__restrict__ Y,
+                                                       const double *__restrict__ sc, double *__restrict__ partial)
+{
+    constexpr int L = 8 / KP;  // lanes per (row, right-hand side)
+    constexpr int G = 256 / 8;
+    const int slot = threadIdx.x >> 3, lane8 = threadIdx.x & 7, j = lane8 / L, sub = lane8 % L;
+    const double sx = sc[j], cy = sc[KP + j];
+    const int chunk = (gridDim.x / 8) * G, lr = (blockIdx.x / 8) * G + slot;
+    double sq = 0;
+    for (int base = (blockIdx.x % 8) * chunk; base < rows; base += 8 * chunk) {
+        const int r = base + lr;
+        if (r >= rows) break;
+        const int a = rs[r], b = rs[r + 1];
+        double acc0 = 0, acc1 = 0;
+        int q = a + sub;
+        for (; q + 3 * L < b; q += 4 * L) {
+            const int c0 = ci[q], c1 = ci[q + L], c2 = ci[q + 2 * L], c3 = ci[q + 3 * L];
+            const float v0 = va[q], v1 = va[q + L], v2 = va[q + 2 * L], v3 = va[q + 3 * L];
+            const T x0 = X[(size_t)c0 * KP + j], x1 = X[(size_t)c1 * KP + j];
+            const T x2 = X[(size_t)c2 * KP + j], x3 = X[(size_t)c3 * KP + j];
+            acc0 += (double)v0 * (double)x0;
+            acc1 += (double)v1 * (double)x1;
+            acc0 += (double)v2 * (double)x2;
+            acc1 += (double)v3 * (double)x3;
+        }
+        for (; q < b; q += L) acc0 += (double)va[q] * (double)X[(size_t)ci[q] * KP + j];
+        double acc = acc0 + acc1;
+#pragma unroll
+        for (int m = 1; m < L; m <<= 1) acc += __shfl_xor(acc, m, 8);
+        if (sub == 0) {
+            const size_t at = (size_t)r * KP + j;
+            const double old = cy != 0.0 ? cy * (double)Y[at] : 0.0;
+            const T y = (T)(sx * acc - old);
+            Y[at] = y;
+            sq += (double)y * (double)y;
+        }
+    }
+    // per right-hand side: the block's slots in a fixed tree, then its lanes in order
+    __shared__ double red[256];
+    red[threadIdx.x] = sq;
+    __syncthreads();
+    for (int s2 = G / 2; s2 > 0; s2 >>= 1) {
+        if (slot < s2) red[threadIdx.x] += red[threadIdx.x + s2 * 8];
+        __syncthreads();
+    }
+    if (threadIdx.x < KP) {
+        double t = 0;
+        for (int u = 0; u < L; u++) t += red[threadIdx.x * L + u];
+        partial[(size_t)blockIdx.x * KP + threadIdx.x] = t;
+    }
+}
+
+// One product: lsqr_spmm, or lsqr_spmm_split below 8 right-hand sides.
+template <int KP, typename T>
+void spmm(hipStream_t st, int rows, const int32_t *rs, const int32_t *ci, const float *va, const T *X, T *Y,
+          const double *sc, double *partial)
+{
+    if constexpr (KP < 8) lsqr_spmm_split<KP, T><<<kSpmmBlocks, 256, 0, st>>>(rows, rs, ci, va, X, Y, sc, partial);
+    else lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(rows, rs, ci, va, X, Y, sc, partial);
+}
+
 // x += cx w ; w = cv V + cw w over n rows; squares of the new w per block (for the next ddnorm).
 template <int KP, typename T>
 __global__ __launch_bounds__(256) void lsqr_xw(int n, const T *__restrict__ V, T *__restrict__ W,
@@ -749,9 +817,9 @@ struct FastLsqr {
 template <int KP, typename T>
 void fast_iteration(ts_matrix &A, FastLsqr<T> &F, hipStream_t st, bool with_hist)
 {
-    lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.rows, A.rs.p, A.ci.p, A.va32.p, F.V.p, F.U.p, F.sc_a.p, F.pu.p);
+    spmm<KP, T>(st, A.rows, A.rs.p, A.ci.p, A.va32.p, F.V.p, F.U.p, F.sc_a.p, F.pu.p);
     lsqr_fin_u<<<1, kFinThreads, 0, st>>>(F.k, KP, F.S.p, F.pu.p, kSpmmBlocks, F.sc_t.p);
-    lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.cols, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
+    spmm<KP, T>(st, A.cols, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
     lsqr_fin_v<<<1, kFinThreads, 0, st>>>(F.k, KP, F.S.p, F.pv.p, F.pw.p, kSpmmBlocks, kSpmmBlocks, F.sc_a.p,
                                  with_hist ? F.hist.p : nullptr, F.max_it);
     lsqr_xw<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.cols, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
@@ -815,7 +883,7 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     for (int j = 0; j < KP; j++) { sc[j] = S[j].done ? 0.0 : S[j].su; sc[KP + j] = 0; }
     TS_HIP(hipMemcpyAsync(F.sc_t.p, sc.data(), 2 * KP * 8, hipMemcpyHostToDevice, st));
     // v = A^T u, alpha, w = v
-    lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(n, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
+    spmm<KP, T>(st, n, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
     lsqr_fin_init<<<1, kFinThreads, 0, st>>>(k, KP, F.S.p, F.pv.p, kSpmmBlocks, F.sc_a.p);
     lsqr_xw<KP, T><<<kSpmmBlocks, 256, 0, st>>>(n, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
     TS_HIP(hipGetLastError());
