@@ -106,7 +106,7 @@ def test_plain_stop_rules_follow_the_oracle(device, oracle):
 
 
 @pytest.mark.parametrize("mode,tol", [("fast_fp32", FAST_TOL), ("fast_mixed", 1e-8)])
-@pytest.mark.parametrize("k", [1, 5, 32, 33])
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 32, 33])
 def test_fast_many_rhs_against_fp64_oracle(device, oracle, k, mode, tol):
     """FP32 matrix values: the oracle runs on the same rounded values. FP32 vectors stay within 1e-5,
     FP64 vectors (mixed) within 1e-8 of the FP64 oracle at equal iteration counts."""
