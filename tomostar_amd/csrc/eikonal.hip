@@ -36,13 +36,16 @@ namespace ts {
 constexpr double kSettleTol = 1e-7;  // SphericalEikonal.cs:332 SettleTolerance
 constexpr int kLanes = 64;
 // The mixed form's first-order phase on FP32 times counts a node as moved when it dropped by more than
-// the settle tolerance and by more than this many FP32 ulps of its time: an FP32 time of tens of seconds
-// cannot move by less than an ulp (~2e-6 s), so without the ulp floor the phase runs an extra round on
-// rounding-level drops. The phase only seeds the FP64 second-order phase, which converges on its own:
-// measured on 400 fields of 201 x 201 x 101, 4 ulps end the phase one round earlier (5 instead of 6)
-// with the result's distance to the FP64 form unchanged.
+// the settle tolerance and by more than this many FP32 ulps of its time (4096 ulps: 4.9e-4 of the
+// time). The phase only seeds the FP64 second-order phase, which lowers every time from there until its
+// own rounds settle at 1e-7 s: times only ever drop, so a first-order phase stopped early leaves times
+// above its fixed point, and the second-order phase takes them down the rest of the way. Measured on
+// 400 fields of 201 x 201 x 101 (first-order rounds, solve seconds, largest relative distance to the
+// FP64 form over every tenth field): 4 ulps (5, 1.089 s, 4.571e-6), 256 (4, 1.068 s, 4.571e-6), 4096
+// (3, 1.029 s, 4.571e-6), 16384 (3, 1.031 s, 4.571e-6), 65536 (3, 1.014 s, 2.2e-5) - the second-order
+// phase takes 6 rounds in each case; from 65536 ulps on its result moves.
 #ifndef TS_FP32_SETTLE_ULPS
-#define TS_FP32_SETTLE_ULPS 4
+#define TS_FP32_SETTLE_ULPS 4096
 #endif
 
 inline int sweep_round_cap(const Grid &g)  // SphericalEikonal.cs:335
