@@ -37,7 +37,7 @@ BYTES_PER_UPDATE_FP64 = 20.0
 # the figures are summarised in profiles/eik_fast_plane_batched_400x201x201x101.md.
 TRAFFIC_BYTES_PER_LAUNCH: dict = {
     "eik_fast_plane<double>": 9.475e7,
-    "eik_fast_plane<float>": 3.516e7,
+    "eik_fast_plane<float>": 5.284e7,
 }
 
 
