@@ -48,6 +48,7 @@ struct MgChunk {
     std::vector<int> seg_off, seg_blocks;
     dbuf<MgSeg> segs;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    double graph_wake[2] = {0, 0};        // the wake tolerance each graph was captured with
 };
 
 struct ts_gridbatch {
@@ -74,6 +75,7 @@ struct ts_gridbatch {
     std::vector<int> tuned;               // nodes per thread chosen per size class
     double seconds = 0, sweep_seconds = 0;
     int last_mode = -1;
+    double wake_tol[2] = {0, 0};          // MgArgs::wake_tol of the running FP32 / FP64 sweeps
     static constexpr int kMaxChunks = 4;
     MgChunk chunk[kMaxChunks];
     int nchunks = 0;                      // > 1: the problems sweep in chunks side by side
@@ -97,6 +99,13 @@ namespace ts {
 namespace {
 
 constexpr double kMgSettleTol = 1e-7;  // SphericalEikonal.cs:332
+// The mixed form's first-order phase on FP32 times: a node has moved when it dropped by more than the
+// settle tolerance and by more than this many FP32 ulps of its time, the batched form's rule (its
+// second-order phase takes the times the rest of the way down).
+constexpr float kMgFp32SettleUlps = 4096;
+// The mixed form's second-order phase wakes a node's readers only on drops above this (s), as the
+// batched form's does; the FP64 form wakes them on every drop, as Relax does.
+constexpr double kMgMixedWakeTol = 1e-10;
 constexpr int kMgBlock = 256;
 
 inline int mg_round_cap(const Grid &g)  // SphericalEikonal.cs:335
@@ -115,6 +124,7 @@ struct MgArgs {
     const uint32_t *planes;
     const int32_t *phase;
     int32_t *changed;
+    double wake_tol;  // a drop wakes the node's readers when larger than this (0: any drop)
 };
 
 template <typename R>
@@ -156,7 +166,13 @@ __device__ inline void mg_update(const MgArgs<R> &A, const MgProblem &P, int q, 
     }
     if (!(next < cur)) return;
     Tp[node] = next;
-    if (!isfinite(cur) || (double)(cur - next) > kMgSettleTol) atomicOr(&A.changed[q], 1);
+    bool moved;
+    if constexpr (sizeof(R) == 4)
+        moved = !isfinite(cur) || (cur - next) > fmaxf((float)kMgSettleTol, kMgFp32SettleUlps * 1.1920929e-7f * cur);
+    else
+        moved = !isfinite(cur) || (double)(cur - next) > kMgSettleTol;
+    if (moved) atomicOr(&A.changed[q], 1);
+    if (isfinite(cur) && !((double)(cur - next) > A.wake_tol)) return;
     uint8_t *d = A.dirty + P.node_base;
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
@@ -354,6 +370,7 @@ MgArgs<R> mg_args(ts_gridbatch &b)
     A.planes = b.planes.p;
     A.phase = b.phase.p;
     A.changed = b.changed.p;
+    A.wake_tol = b.wake_tol[sizeof(R) == 8];
     return A;
 }
 
@@ -388,8 +405,14 @@ void mg_round(ts_gridbatch &b, hipStream_t st, int hold)
 template <typename R>
 hipGraphExec_t mg_chunk_graph(ts_gridbatch &b, int c, hipStream_t cs)
 {
-    hipGraphExec_t &ge = b.chunk[c].graph[sizeof(R) == 8];
+    const int w = sizeof(R) == 8;
+    hipGraphExec_t &ge = b.chunk[c].graph[w];
+    if (ge && b.chunk[c].graph_wake[w] != b.wake_tol[w]) {  // captured with another tolerance
+        TS_HIP(hipGraphExecDestroy(ge));
+        ge = nullptr;
+    }
     if (ge) return ge;
+    b.chunk[c].graph_wake[w] = b.wake_tol[w];
     hipGraph_t gr;
     TS_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     mg_planes<R>(b, cs, b.chunk[c].seg_off, b.chunk[c].seg_blocks, b.chunk[c].segs.p);
@@ -489,6 +512,8 @@ void mg_solve(ts_gridbatch &b, int mode, const volatile int32_t *cancel)
     int max_rounds = 0;
     for (const Grid &g : b.grids) max_rounds = std::max(max_rounds, 2 * mg_round_cap(g) + 2);
     b.sweep_seconds = 0;
+    b.wake_tol[0] = 0.0;
+    b.wake_tol[1] = mixed ? kMgMixedWakeTol : 0.0;
     mg_start(b, mixed, st);
     if (mixed) {
         mg_rounds<float>(b, cancel, 1, max_rounds, h_status, d_base.p);
