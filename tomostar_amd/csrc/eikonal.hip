@@ -1022,15 +1022,18 @@ void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel
 // reaches it to rounding), then the second-order phase in FP64 from the widened FP32 times - the
 // second-order promotion amplifies storage rounding (an FP32 second-order phase drifts by up to
 // 4e-4 of the time on a 0.5 km grid), so it keeps FP64 times.
-// The mixed form's second-order phase wakes a node's readers only on drops above 1e-10 s (1/1000 of the
+// The mixed form's second-order phase wakes a node's readers only on drops above 1e-8 s (1/10 of the
 // settle tolerance; TOMOSTAR_MIXED_WAKE_TOL overrides): the phase starts near its fixed point, and
 // without the threshold rounding-level drops keep almost every node dirty in every round. Measured on
-// 400 fields of 201 x 201 x 101: the phase takes 0.76 s instead of 1.25 s, and the times move by 2e-12 s
-// on average.
+// 400 fields of 201 x 201 x 101 (solve seconds; mean and largest change of the times against the
+// 1e-10 s threshold over every tenth field): 1e-10 s 1.031 s; 1e-9 s 1.005 s, 6.2e-12 / 8.8e-9 s;
+// 1e-8 s 0.975 s, 5.9e-11 / 1.1e-7 s; 3e-8 s 0.962 s, 2.1e-10 / 8.1e-7 s. At 1e-8 s no time moves by
+// more than the settle tolerance, and the largest distance to the FP64 form (4.571e-6 relative) is
+// unchanged. Without any threshold the phase took 1.25 s against 0.76 s at 1e-10 s.
 double mixed_wake_tol()
 {
     if (const char *e = getenv("TOMOSTAR_MIXED_WAKE_TOL")) return std::max(0.0, atof(e));
-    return 1e-10;
+    return 1e-8;
 }
 
 void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool count)

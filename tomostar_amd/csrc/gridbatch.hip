@@ -105,7 +105,7 @@ constexpr double kMgSettleTol = 1e-7;  // SphericalEikonal.cs:332
 constexpr float kMgFp32SettleUlps = 4096;
 // The mixed form's second-order phase wakes a node's readers only on drops above this (s), as the
 // batched form's does; the FP64 form wakes them on every drop, as Relax does.
-constexpr double kMgMixedWakeTol = 1e-10;
+constexpr double kMgMixedWakeTol = 1e-8;
 constexpr int kMgBlock = 256;
 
 inline int mg_round_cap(const Grid &g)  // SphericalEikonal.cs:335
