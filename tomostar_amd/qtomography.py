@@ -10,8 +10,8 @@ correction right-hand side with the device product, runs LSQR, and predicts ever
 (RejectOutliers, :412-432), Run (:434-505) and TradeOff (:520-554).
 
 Scope: a plain node grid (no adaptive mesh or parameter lattice), Laplacian smoothing or unweighted
-first differences (no reweighting passes), no formal resolution, rays inside the grid (no outside
-time).
+first differences (no reweighting passes), rays inside the grid (no outside time). The formal
+resolution of the last system (:480-497) runs on the device when the settings ask for it.
 """
 from __future__ import annotations
 
