@@ -875,21 +875,22 @@ void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count,
     else launch_round_order<R, false>(t, T, st, hold, order);
 }
 
-// How round `round` of a run reads the dirty flags. A solve starts with only the seeds' readers dirty,
-// so flags first from round 0; the mixed form's second-order phase starts with every node dirty and
-// most of them clean after its first round, so from round 1 (TOMOSTAR_FP64_FLAG_FIRST overrides). Later,
-// when the sweeps mostly confirm, a wave reads kSparseNodes flags at once - in runs of one phase: in
-// the FP64 form a field's second-order phase starts in mid-run with every node of its group dirty
-// again. A first-order phase from the seeds reads them so from round 2 (TOMOSTAR_SPARSE_FROM
-// overrides), the mixed form's second-order phase from the round after its first flag-first round
-// (TOMOSTAR_SECOND_SPARSE_FROM overrides). Measured on 400 fields of 201 x 201 x 101: 4 flags per wave
-// from these rounds on took the solve from 1.26 s to 1.20 s; 16 or 32 flags per wave are slower
-// (fewer waves left to hide the latency of the dirty nodes). The first-order phase's round 2 as flag
-// first adds 12 ms; the second-order phase sparse from round 1, 2 or 3 takes 0.965, 0.965 and 0.974 s
-// per solve (wake threshold 1e-8 s).
+// How round `round` of a run reads the dirty flags:
+// - the mixed form's first-order phase issues every load of every update at once in rounds 0-1 and
+//   reads four flags per wave from round 2 (TOMOSTAR_SPARSE_FROM overrides the 2). Measured on 400
+//   fields of 201 x 201 x 101: the phase takes 0.322 s against 0.333 s with rounds 0-1 flag first and
+//   0.347 s with round 2 also all loads at once; round 2 flag first adds 12 ms;
+// - the mixed form's second-order phase starts with every node dirty and most of them clean after its
+//   first round: all loads at once in round 0, flag first from round 1 (TOMOSTAR_FP64_FLAG_FIRST
+//   overrides), four flags per wave from the round after (TOMOSTAR_SECOND_SPARSE_FROM overrides).
+//   Sparse from round 1, 2 or 3 takes the solve 0.965, 0.965 and 0.974 s (wake threshold 1e-8 s);
+// - the FP64 form reads the flag first in every round: a field's second-order phase starts in mid-run
+//   with every node of its group dirty again.
+// Four flags per wave first took the solve from 1.26 s to 1.20 s; 16 or 32 flags per wave are slower
+// (fewer waves left to hide the latency of the dirty nodes).
 int round_order(bool all_dirty, bool one_phase, int round)
 {
-    int first = 0;
+    int first = one_phase ? 2 : 0;
     if (all_dirty) {
         first = 1;
         if (const char *e = getenv("TOMOSTAR_FP64_FLAG_FIRST")) first = std::max(0, atoi(e));
