@@ -219,3 +219,39 @@ def test_full_size_batched_mixed_against_fp64(device):
     print(f"max relative error of the mixed form over {len(picks)} fields: {err:.3e}")
     assert err < FP32_TOL
     ts.release()
+
+
+def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
+    """How a round reads the dirty flags (every load at once, the flag first, four flags per wave) and
+    how many streams the groups sweep on change when loads are issued, not which nodes update nor their
+    arithmetic: no node of a plane reads or wakes another node of the same plane, and the groups are
+    independent fields. The mixed form's times are bit-identical under every schedule, also when the
+    stream count changes between solves of one table set (its graphs then cover other groups)."""
+    from tomostar_amd import eikonal as E
+    from tomostar_amd import workloads as W
+
+    wl = _checker_case(7, nx=41, ny=37, nz=21, nst=70)  # 140 fields: three groups of 64 lanes
+    seeds, mof = W.station_seed_sets(wl, E.point_source)
+    ts = E.TableSet(device, wl.grid, [wl.slowness_p, wl.slowness_s], seeds, mof)
+    rc, why = ts.solve("fast_fp32")
+    assert rc == 0, why
+    base = ts.read()
+    assert np.isfinite(base).any()
+    schedules = [
+        {"TOMOSTAR_SPARSE_FROM": "99", "TOMOSTAR_SECOND_SPARSE_FROM": "99"},  # no four-flag rounds
+        {"TOMOSTAR_FP64_FLAG_FIRST": "0"},  # second-order phase flag first from its round 0
+        {"TOMOSTAR_GROUP_STREAMS": "1"},
+        {"TOMOSTAR_GROUP_STREAMS": "2"},  # chunk 1 now holds groups 1-2 (with 3 streams: group 1)
+        {},
+    ]
+    for env in schedules:
+        for k in ("TOMOSTAR_SPARSE_FROM", "TOMOSTAR_SECOND_SPARSE_FROM", "TOMOSTAR_FP64_FLAG_FIRST",
+                  "TOMOSTAR_GROUP_STREAMS"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        print(f"schedule {env}", flush=True)
+        rc, why = ts.solve("fast_fp32")
+        assert rc == 0, why
+        assert np.array_equal(ts.read(), base, equal_nan=True), env
+    ts.release()

@@ -572,6 +572,7 @@ struct ts_tableset {
     hipEvent_t gfork = nullptr, gjoin[kMaxStreams] = {};
     hipGraphExec_t cgraphs[2][2][3][2][kMaxStreams] = {};
     double cgraph_wake_tol[2][2][3][2][kMaxStreams] = {};
+    int cgraph_groups[2][2][3][2][kMaxStreams][2] = {};  // the groups [first, first + count) each swept
     std::mutex gate;
     ~ts_tableset()
     {
@@ -936,12 +937,16 @@ hipGraphExec_t chunk_graph(ts_tableset &t, R *T, hipStream_t st, bool count, int
     const int woken = t.wake_tol[w] > 0;
     hipGraphExec_t &ge = t.cgraphs[w][count][order][woken][c];
     double &captured = t.cgraph_wake_tol[w][count][order][woken][c];
-    if (ge && captured != t.wake_tol[w]) {
+    int *range = t.cgraph_groups[w][count][order][woken][c];
+    // captured with another tolerance, or for other groups (the stream count changed between solves)
+    if (ge && (captured != t.wake_tol[w] || range[0] != g0 || range[1] != gc)) {
         TS_HIP(hipGraphExecDestroy(ge));
         ge = nullptr;
     }
     if (ge) return ge;
     captured = t.wake_tol[w];
+    range[0] = g0;
+    range[1] = gc;
     hipGraph_t gr;
     TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     launch_planes_as<R>(t, T, st, count, order, g0, gc);
