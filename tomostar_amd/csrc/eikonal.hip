@@ -758,7 +758,7 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
 constexpr int kWPB = 4;
 constexpr int kSparse = 2;  // round order after kFlagWithLoads, kFlagFirst: kSparseNodes flags per wave, then the dirty nodes
 #ifndef TS_SPARSE_NODES
-#define TS_SPARSE_NODES 4
+#define TS_SPARSE_NODES 2
 #endif
 constexpr int kSparseNodes = TS_SPARSE_NODES;
 
@@ -878,17 +878,18 @@ void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count,
 
 // How round `round` of a run reads the dirty flags:
 // - the mixed form's first-order phase issues every load of every update at once in rounds 0-1 and
-//   reads four flags per wave from round 2 (TOMOSTAR_SPARSE_FROM overrides the 2). Measured on 400
+//   reads kSparseNodes flags per wave from round 2 (TOMOSTAR_SPARSE_FROM overrides the 2). Measured on 400
 //   fields of 201 x 201 x 101: the phase takes 0.322 s against 0.333 s with rounds 0-1 flag first and
 //   0.347 s with round 2 also all loads at once; round 2 flag first adds 12 ms;
 // - the mixed form's second-order phase starts with every node dirty and most of them clean after its
 //   first round: all loads at once in round 0, flag first from round 1 (TOMOSTAR_FP64_FLAG_FIRST
-//   overrides), four flags per wave from the round after (TOMOSTAR_SECOND_SPARSE_FROM overrides).
+//   overrides), kSparseNodes flags per wave from the round after (TOMOSTAR_SECOND_SPARSE_FROM overrides).
 //   Sparse from round 1, 2 or 3 takes the solve 0.965, 0.965 and 0.974 s (wake threshold 1e-8 s);
 // - the FP64 form reads the flag first in every round: a field's second-order phase starts in mid-run
 //   with every node of its group dirty again.
 // Four flags per wave first took the solve from 1.26 s to 1.20 s; 16 or 32 flags per wave are slower
-// (fewer waves left to hide the latency of the dirty nodes).
+// (fewer waves left to hide the latency of the dirty nodes). With the rounds above, 1, 2, 4 and 8 flags
+// per wave (kSparseNodes) take the solve 0.986, 0.946, 0.959 and 1.024 s.
 int round_order(bool all_dirty, bool one_phase, int round)
 {
     int first = one_phase ? 2 : 0;
