@@ -238,8 +238,9 @@ def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
     base = ts.read()
     assert np.isfinite(base).any()
     schedules = [
-        {"TOMOSTAR_SPARSE_FROM": "99", "TOMOSTAR_SECOND_SPARSE_FROM": "99"},  # no four-flag rounds
-        {"TOMOSTAR_FP64_FLAG_FIRST": "0"},  # second-order phase flag first from its round 0
+        {"TOMOSTAR_SPARSE_FROM": "99", "TOMOSTAR_SECOND_SPARSE_FROM": "99"},  # flag first, no sparse rounds
+        {"TOMOSTAR_FP64_FLAG_FIRST": "0"},  # second-order phase sparse from its round 0 (no all-loads round)
+        {"TOMOSTAR_SECOND_SPARSE_FROM": "3"},  # second-order rounds 1-2 flag first
         {"TOMOSTAR_GROUP_STREAMS": "1"},
         {"TOMOSTAR_GROUP_STREAMS": "2"},  # chunk 1 now holds groups 1-2 (with 3 streams: group 1)
         {},

@@ -882,9 +882,10 @@ void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count,
 //   fields of 201 x 201 x 101: the phase takes 0.322 s against 0.333 s with rounds 0-1 flag first and
 //   0.347 s with round 2 also all loads at once; round 2 flag first adds 12 ms;
 // - the mixed form's second-order phase starts with every node dirty and most of them clean after its
-//   first round: all loads at once in round 0, flag first from round 1 (TOMOSTAR_FP64_FLAG_FIRST
-//   overrides), kSparseNodes flags per wave from the round after (TOMOSTAR_SECOND_SPARSE_FROM overrides).
-//   Sparse from round 1, 2 or 3 takes the solve 0.965, 0.965 and 0.974 s (wake threshold 1e-8 s);
+//   first round: all loads at once in round 0 (TOMOSTAR_FP64_FLAG_FIRST moves the end of the all-loads
+//   rounds), kSparseNodes flags per wave from round 1 (TOMOSTAR_SECOND_SPARSE_FROM overrides; rounds
+//   between the two read the flag first). Sparse from round 1, 2 or 3 takes the solve 0.938, 0.946 and
+//   0.957 s at two flags per wave (0.965, 0.965 and 0.974 s at four);
 // - the FP64 form reads the flag first in every round: a field's second-order phase starts in mid-run
 //   with every node of its group dirty again.
 // Four flags per wave first took the solve from 1.26 s to 1.20 s; 16 or 32 flags per wave are slower
@@ -897,7 +898,7 @@ int round_order(bool all_dirty, bool one_phase, int round)
         first = 1;
         if (const char *e = getenv("TOMOSTAR_FP64_FLAG_FIRST")) first = std::max(0, atoi(e));
     }
-    int sparse = all_dirty ? first + 1 : 2;
+    int sparse = all_dirty ? first : 2;
     if (const char *e = getenv(all_dirty ? "TOMOSTAR_SECOND_SPARSE_FROM" : "TOMOSTAR_SPARSE_FROM"))
         sparse = std::max(0, atoi(e));
     return round < first ? kFlagWithLoads : round < sparse || !one_phase ? kFlagFirst : kSparse;
