@@ -877,10 +877,10 @@ void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count,
 }
 
 // How round `round` of a run reads the dirty flags:
-// - the mixed form's first-order phase issues every load of every update at once in rounds 0-1 and
-//   reads kSparseNodes flags per wave from round 2 (TOMOSTAR_SPARSE_FROM overrides the 2). Measured on 400
-//   fields of 201 x 201 x 101: the phase takes 0.322 s against 0.333 s with rounds 0-1 flag first and
-//   0.347 s with round 2 also all loads at once; round 2 flag first adds 12 ms;
+// - the mixed form's first-order phase reads kSparseNodes flags per wave from round 0
+//   (TOMOSTAR_SPARSE_FROM overrides; rounds before it read the flag first). Measured on 400 fields of
+//   201 x 201 x 101, the phase takes 0.306 s; 0.313 s with round 0 issuing all loads at once, 0.321 s
+//   with rounds 0-1 so;
 // - the mixed form's second-order phase starts with every node dirty and most of them clean after its
 //   first round: all loads at once in round 0 (TOMOSTAR_FP64_FLAG_FIRST moves the end of the all-loads
 //   rounds), kSparseNodes flags per wave from round 1 (TOMOSTAR_SECOND_SPARSE_FROM overrides; rounds
@@ -888,17 +888,17 @@ void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count,
 //   0.957 s at two flags per wave (0.965, 0.965 and 0.974 s at four);
 // - the FP64 form reads the flag first in every round: a field's second-order phase starts in mid-run
 //   with every node of its group dirty again.
-// Four flags per wave first took the solve from 1.26 s to 1.20 s; 16 or 32 flags per wave are slower
-// (fewer waves left to hide the latency of the dirty nodes). With the rounds above, 1, 2, 4 and 8 flags
-// per wave (kSparseNodes) take the solve 0.986, 0.946, 0.959 and 1.024 s.
+// Sparse rounds first took the solve from 1.26 s to 1.20 s; with the rounds above, 1, 2, 4 and 8 flags
+// per wave (kSparseNodes) take it 0.986, 0.946, 0.959 and 1.024 s (16 or 32: slower still; fewer waves
+// are left to hide the latency of the dirty nodes).
 int round_order(bool all_dirty, bool one_phase, int round)
 {
-    int first = one_phase ? 2 : 0;
+    int first = 0;
     if (all_dirty) {
         first = 1;
         if (const char *e = getenv("TOMOSTAR_FP64_FLAG_FIRST")) first = std::max(0, atoi(e));
     }
-    int sparse = all_dirty ? first : 2;
+    int sparse = all_dirty ? first : 0;
     if (const char *e = getenv(all_dirty ? "TOMOSTAR_SECOND_SPARSE_FROM" : "TOMOSTAR_SPARSE_FROM"))
         sparse = std::max(0, atoi(e));
     return round < first ? kFlagWithLoads : round < sparse || !one_phase ? kFlagFirst : kSparse;
