@@ -222,7 +222,7 @@ def test_full_size_batched_mixed_against_fp64(device):
 
 
 def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
-    """How a round reads the dirty flags (every load at once, the flag first, four flags per wave) and
+    """How a round reads the dirty flags (every load at once, the flag first, a few flags per wave) and
     how many streams the groups sweep on change when loads are issued, not which nodes update nor their
     arithmetic: no node of a plane reads or wakes another node of the same plane, and the groups are
     independent fields. The mixed form's times are bit-identical under every schedule, also when the
