@@ -45,13 +45,34 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu() -> dict:
+    """The host's CPUs: nproc, the CPUs this process may run on (sched_getaffinity), the threads the
+    CPU baselines use, and the CPU model. The threads are every CPU of the affinity set, capped by
+    OMP_NUM_THREADS when it is set: the GPU box gives one GPU's jobs a share of its cores through it."""
+    avail = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(avail, int(share)) if share.isdigit() and int(share) > 0 else avail
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count() or 1, "affinity": avail, "threads": threads, "model": model,
+            "omp_num_threads": share or None}
+
+
 def cpu_baseline(wl, seeds, mof, budget_s: float = 20.0):
     """The reference's CPU path restated (fast marching, one table per core) on a bounded sample."""
     from oracle import oracle as O
 
     g = wl.grid
     og = O.griddef(g.min_lon, g.max_lon, g.min_lat, g.max_lat, g.min_depth, g.max_depth, g.nx, g.ny, g.nz)
-    cores = min(os.cpu_count() or 1, 16)
+    host = host_cpu()
+    cores = host["threads"]
     models = [wl.slowness_p, wl.slowness_s]
     # probe one field on one core, then size the sample to ~budget_s of wall time on all cores
     t0 = time.perf_counter()
@@ -63,6 +84,8 @@ def cpu_baseline(wl, seeds, mof, budget_s: float = 20.0):
     O.fmm_batch(og, [models[m] for m in mof[:nf]], seeds[:nf], threads=cores)
     wall = time.perf_counter() - t0
     return {"value": nf * g.count / wall / 1e9, "unit": "Gnode/s", "cores": cores, "kind": "port",
+            "nproc": host["nproc"], "affinity_cpus": host["affinity"], "omp_num_threads": host["omp_num_threads"],
+            "cpu_model": host["model"],
             "sample": f"{nf} of {len(seeds)} fields, fast marching (SphericalEikonal.Solve restated in C), "
                       f"one field per core, {wall:.1f} s"}
 
@@ -236,7 +259,7 @@ def q_leg(dev, wl, cpu: bool):
 
         g = wl.grid
         og = O.griddef(g.min_lon, g.max_lon, g.min_lat, g.max_lat, g.min_depth, g.max_depth, g.nx, g.ny, g.nz)
-        cores = min(os.cpu_count() or 1, 16)
+        cores = host_cpu()["threads"]
         rows = np.flatnonzero(c.rays.counts >= 0)
         starts, ends = c.rays.row_start[rows], c.rays.row_start[rows + 1]
         rs = np.concatenate([[0], np.cumsum(ends - starts)]).astype(np.int64)
@@ -385,7 +408,7 @@ def cpu_baseline_consumers(r, cons):
     from oracle import oracle as O
     from tomostar_amd import resolution as R
 
-    cores = min(os.cpu_count() or 1, 16)
+    cores = host_cpu()["threads"]
     rs, ci, va = r.system.read()
     z = R.probe_signs(r.columns, 2, seed=1)
     t0 = time.perf_counter()
@@ -407,7 +430,7 @@ def cpu_baseline_iteration(wl, r):
 
     g = wl.grid
     og = O.griddef(g.min_lon, g.max_lon, g.min_lat, g.max_lat, g.min_depth, g.max_depth, g.nx, g.ny, g.nz)
-    cores = min(os.cpu_count() or 1, 16)
+    cores = host_cpu()["threads"]
     sP, sS = It.starting_model(wl)
     sec = {}
     t0 = time.perf_counter()

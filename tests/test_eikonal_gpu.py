@@ -256,3 +256,42 @@ def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
         assert rc == 0, why
         assert np.array_equal(ts.read(), base, equal_nan=True), env
     ts.release()
+
+
+def _suggest_batch_rule(nodes, global_mem):
+    """EikonalOpenCl.SuggestBatch (EikonalOpenCl.cs:121-132) with Real = 8 (an FP64 device) and MaxAlloc =
+    the device's memory (HIP sets no separate allocation limit); C#'s long division truncates."""
+    real = 8
+    per_field = nodes * real + 2 * nodes
+    budget = int(global_mem * 0.5) - 2 * nodes * real
+    by_alloc = global_mem // max(1, nodes * real)
+    by_host = (768 << 20) // max(1, nodes * 8)
+    enough = max(4, (128 << 20) // max(1, nodes))
+    return int(np.clip(min(min(budget // max(1, per_field), by_alloc), min(by_host, enough)), 0, 32))
+
+
+def test_suggest_batch_follows_the_reference_rule(device):
+    import re
+
+    from tomostar_amd import _lib
+
+    L = _lib.load()
+    m = re.search(r"([0-9.]+) GB", device.info)
+    assert m, device.info
+    global_mem = float(m.group(1)) * 1e9
+    assert global_mem > 100e9  # on this device the host and batch terms bind before the memory ones
+    for nodes in (1, 1000, 115_351, 3_000_000, 4_080_501, 10_000_000, 33_554_432, 50_000_000, 100_663_296,
+                  200_000_000, 2_000_000_000):
+        want = _suggest_batch_rule(nodes, int(global_mem))
+        # the memory terms are far from binding, so the device's exact byte count cannot change the answer
+        assert want == _suggest_batch_rule(nodes, int(global_mem * 1.001)) == _suggest_batch_rule(nodes, int(global_mem * 0.999))
+        assert L.ts_eik_suggest_batch(device.handle, nodes) == want, nodes
+    assert L.ts_eik_suggest_batch(device.handle, 4_080_501) == 24  # 768 MiB of FP64 host buffers: 24 fields of 201 x 201 x 101
+
+
+def test_host_seed_and_metric_rows_on_the_device_box(oracle):
+    """GridMetric and PointSource against the oracle to the last bit, run where the GPU tests run."""
+    import test_abi
+
+    test_abi.test_grid_metric_and_point_source_match_oracle(oracle)
+    test_abi.test_point_source_at_faces_and_corners(oracle)
