@@ -212,6 +212,15 @@ int ts_matrix_read_transpose(ts_matrix *a, int32_t *row_start, int32_t *column, 
  * ResolutionDiagonal.Estimate's A z and TradeOff's G x, Tomography/ResolutionDiagonal.cs:48,
  * TravelTimeTomography.cs:599). x[k][cols or rows], y[k][rows or cols], in the given orders. */
 int ts_matrix_multiply(ts_matrix *a, int k, const double *x, double *y, int transpose, char *reason, int reason_len);
+/* ts_matrix_multiply without any device allocation: no counterpart in the reference. It is for callers
+ * that cannot allocate device memory during the call (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The input and output are staged in one 64 MiB array per GPU,
+ * allocated with the library's code object and held until the process ends, so a call needs
+ * 2 x (rows + cols) <= 8,388,608 doubles; larger matrices are rejected with TS_EINVAL and a reason.
+ * Results equal ts_matrix_multiply's bit for bit (the same kernels). Calls on one GPU share its array:
+ * the library makes them take turns, and the array keeps the last call's contents. */
+int ts_matrix_multiply_noalloc(ts_matrix *a, int k, const double *x, double *y, int transpose, char *reason,
+                               int reason_len);
 void ts_matrix_release(ts_matrix *a);
 
 /* ---------------------------------------------------------------- the linear system

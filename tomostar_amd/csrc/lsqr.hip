@@ -1206,6 +1206,71 @@ int ts_matrix_create(ts_context *ctx, int rows, int cols, const int32_t *row_sta
     });
 }
 
+}  // extern "C"
+
+namespace ts {
+
+// y = A x or A^T x for k vectors through four device vectors the caller provides: hin[nin] (the input
+// as given), din[nin] (in the stored order), dout[nout] (in the stored order), hout[nout] (as given).
+// Runs on the matrix's stream and waits for each vector's copy back.
+static void matrix_multiply_with(ts_matrix *a, int k, const double *x, double *y, int transpose, double *hin,
+                                 double *din, double *dout, double *hout)
+{
+    hipStream_t st = a->ctx->stream;
+    const int m = a->rows, n = a->cols;
+    const int nin = transpose ? m : n, nout = transpose ? n : m;
+    // permutation of the input into the stored order and of the output back to the given one
+    const int32_t *pin = transpose ? a->row_perm.p : a->col_perm.p;
+    const int32_t *pout = transpose ? a->col_perm.p : a->row_perm.p;
+    for (int j = 0; j < k; j++) {
+        TS_HIP(hipMemcpyAsync(hin, x + (size_t)j * nin, (size_t)nin * 8, hipMemcpyHostToDevice, st));
+        if (pin) mx_permute_vec<<<grid_for(nin), 256, 0, st>>>(hin, pin, nin, din);
+        const double *xin = pin ? din : hin;
+        double *yout = pout ? dout : hout;
+        if (transpose) {
+            if (a->tva64.p) mx_spmv<double><<<grid_for(n), 256, 0, st>>>(n, a->trs.p, a->tci.p, a->tva64.p, xin, yout);
+            else mx_spmv<float><<<grid_for(n), 256, 0, st>>>(n, a->trs.p, a->tci.p, a->tva32.p, xin, yout);
+        } else {
+            if (a->va64.p) mx_spmv<double><<<grid_for(m), 256, 0, st>>>(m, a->rs.p, a->ci.p, a->va64.p, xin, yout);
+            else mx_spmv<float><<<grid_for(m), 256, 0, st>>>(m, a->rs.p, a->ci.p, a->va32.p, xin, yout);
+        }
+        if (pout) mx_unpermute_vec<<<grid_for(nout), 256, 0, st>>>(dout, pout, nout, hout);
+        TS_HIP(hipGetLastError());
+        TS_HIP(hipMemcpyAsync(y + (size_t)j * nout, hout, (size_t)nout * 8, hipMemcpyDeviceToHost, st));
+        TS_HIP(hipStreamSynchronize(st));
+    }
+}
+
+// The no-allocation product's staging memory: one array per GPU, allocated with the code object when
+// the library loads and held until the process ends. Host code never passes `mul_scratch` itself to a
+// kernel or a copy (in host code the name is only a stand-in); it uses the device address that
+// hipGetSymbolAddress returns for the current GPU, kept per device in mul_scratch_ptr.
+constexpr size_t kMulScratchDoubles = (size_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double mul_scratch[kMulScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex mul_scratch_setup;
+static double *mul_scratch_ptr[kMaxDevices];
+static std::mutex mul_scratch_use[kMaxDevices];  // calls on one GPU take turns at its array
+
+// The device address of this GPU's array, looked up once per device with that device current.
+static double *mul_scratch_on(int device)
+{
+    if (device < 0 || device >= kMaxDevices) fail(TS_EDEVICE, "device %d beyond the %d the library tracks", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(mul_scratch_setup);
+    if (!mul_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        TS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(mul_scratch)));
+        if (!ptr) fail(TS_EDEVICE, "no device address for the product's scratch array");
+        mul_scratch_ptr[device] = (double *)ptr;
+    }
+    return mul_scratch_ptr[device];
+}
+
+}  // namespace ts
+
+extern "C" {
+
 int ts_matrix_multiply(ts_matrix *a, int k, const double *x, double *y, int transpose, char *reason, int reason_len)
 {
     if (!a || k < 0 || (k > 0 && (!x || !y))) {
@@ -1215,30 +1280,37 @@ int ts_matrix_multiply(ts_matrix *a, int k, const double *x, double *y, int tran
     return guarded(reason, reason_len, [&]() -> int {
         TS_HIP(hipSetDevice(a->ctx->device));
         std::lock_guard<std::mutex> lk(a->gate);
-        hipStream_t st = a->ctx->stream;
-        const int m = a->rows, n = a->cols;
-        const int nin = transpose ? m : n, nout = transpose ? n : m;
-        // permutation of the input into the stored order and of the output back to the given one
-        const int32_t *pin = transpose ? a->row_perm.p : a->col_perm.p;
-        const int32_t *pout = transpose ? a->col_perm.p : a->row_perm.p;
+        const int nin = transpose ? a->rows : a->cols, nout = transpose ? a->cols : a->rows;
         dbuf<double> hin(std::max(1, nin)), din(std::max(1, nin)), dout(std::max(1, nout)), hout(std::max(1, nout));
-        for (int j = 0; j < k; j++) {
-            TS_HIP(hipMemcpyAsync(hin.p, x + (size_t)j * nin, (size_t)nin * 8, hipMemcpyHostToDevice, st));
-            if (pin) mx_permute_vec<<<grid_for(nin), 256, 0, st>>>(hin.p, pin, nin, din.p);
-            const double *xin = pin ? din.p : hin.p;
-            double *yout = pout ? dout.p : hout.p;
-            if (transpose) {
-                if (a->tva64.p) mx_spmv<double><<<grid_for(n), 256, 0, st>>>(n, a->trs.p, a->tci.p, a->tva64.p, xin, yout);
-                else mx_spmv<float><<<grid_for(n), 256, 0, st>>>(n, a->trs.p, a->tci.p, a->tva32.p, xin, yout);
-            } else {
-                if (a->va64.p) mx_spmv<double><<<grid_for(m), 256, 0, st>>>(m, a->rs.p, a->ci.p, a->va64.p, xin, yout);
-                else mx_spmv<float><<<grid_for(m), 256, 0, st>>>(m, a->rs.p, a->ci.p, a->va32.p, xin, yout);
-            }
-            if (pout) mx_unpermute_vec<<<grid_for(nout), 256, 0, st>>>(dout.p, pout, nout, hout.p);
-            TS_HIP(hipGetLastError());
-            TS_HIP(hipMemcpyAsync(y + (size_t)j * nout, hout.p, (size_t)nout * 8, hipMemcpyDeviceToHost, st));
-            TS_HIP(hipStreamSynchronize(st));
-        }
+        matrix_multiply_with(a, k, x, y, transpose, hin.p, din.p, dout.p, hout.p);
+        return TS_OK;
+    });
+}
+
+int ts_matrix_multiply_noalloc(ts_matrix *a, int k, const double *x, double *y, int transpose, char *reason,
+                               int reason_len)
+{
+    if (!a || k < 0 || (k > 0 && (!x || !y))) {
+        set_reason(reason, reason_len, "invalid argument");
+        return TS_EINVAL;
+    }
+    const size_t need = 2 * ((size_t)a->rows + (size_t)a->cols);
+    if (need > kMulScratchDoubles) {
+        char msg[200];
+        snprintf(msg, sizeof msg, "the product needs %zu doubles of scratch (2 x (rows + cols)); the limit is %zu (%zu MiB)",
+                 need, kMulScratchDoubles, (kMulScratchDoubles * 8) >> 20);
+        set_reason(reason, reason_len, msg);
+        return TS_EINVAL;
+    }
+    return guarded(reason, reason_len, [&]() -> int {
+        const int dev = a->ctx->device;
+        TS_HIP(hipSetDevice(dev));
+        double *scratch = mul_scratch_on(dev);
+        std::lock_guard<std::mutex> use(mul_scratch_use[dev]);
+        std::lock_guard<std::mutex> lk(a->gate);
+        const size_t nin = transpose ? a->rows : a->cols, nout = transpose ? a->cols : a->rows;
+        double *hin = scratch, *din = hin + nin, *dout = din + nin, *hout = dout + nout;
+        matrix_multiply_with(a, k, x, y, transpose, hin, din, dout, hout);
         return TS_OK;
     });
 }

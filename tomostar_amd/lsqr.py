@@ -56,6 +56,8 @@ def _declare():
     L.ts_matrix_release.argtypes = [vp]
     L.ts_matrix_multiply.restype = i
     L.ts_matrix_multiply.argtypes = [vp, i, vp, vp, i, C.c_char_p, i]
+    L.ts_matrix_multiply_noalloc.restype = i
+    L.ts_matrix_multiply_noalloc.argtypes = [vp, i, vp, vp, i, C.c_char_p, i]
     L.ts_lsqr_solve.restype = i
     L.ts_lsqr_solve.argtypes = [vp, i, vp, vp, i, d, d, d, i, vp, vp, vp, vp, vp, vp, C.c_char_p, i]
     L.ts_hip_selftest_lsqr_fast.restype = i
