@@ -93,6 +93,12 @@ int ts_grid_metric(const ts_grid *grid, double *hlon, double *hlat, double *hr);
 int ts_point_source(const ts_grid *grid, const double *slowness, double lon, double lat, double depth_km,
                     double origin_time, double radius_cells, int32_t *index_out, double *time_out, int cap);
 
+/* The correctly rounded sine, cosine, arcsine and arctangent every position of the library is computed
+ * with, in place of the platform's Math.Sin / Cos / Asin / Atan2 that GeoMath calls (GeoMath.cs:49-80):
+ * out[q] = f(a[q]) for func 0 sin, 1 cos, 2 asin, or atan2(a[q], b[q]) for func 3; on the host when ctx
+ * is NULL, else on the context's device. */
+int ts_crmath_eval(ts_context *ctx, int func, int n, const double *a, const double *b, double *out);
+
 /* ---------------------------------------------------------------- eikonal
  * EikonalOpenCl.SuggestBatch (EikonalOpenCl.cs:121-132): fields per plain batch for a grid. */
 int ts_eik_suggest_batch(ts_context *ctx, int nodes);
@@ -207,6 +213,11 @@ int ts_matrix_read(ts_matrix *a, int32_t *row_start, int32_t *column, double *va
 /* The stored transpose (CsrMatrix.Transpose, SparseMatrix.cs:79-97): row_start[cols+1], column[nnz]
  * (each transposed row's rows ascending) and its FP32 values. */
 int ts_matrix_read_transpose(ts_matrix *a, int32_t *row_start, int32_t *column, float *values);
+/* The order the matrix is stored in on the device (the products and solves visit rows and columns in
+ * it; CsrMatrix keeps the given order, SparseMatrix.cs:24-44): stored row r is given row row_perm[r],
+ * stored column c is given column col_perm[c]; each stored row keeps its entries in the given order.
+ * Either pointer may be NULL. */
+int ts_matrix_stored_order(ts_matrix *a, int32_t *row_perm, int32_t *col_perm);
 /* y = A x (transpose = 0) or y = A^T x (transpose != 0) for k vectors on the device, FP64 sums over the
  * FP64 values when kept, else the FP32 ones (CsrMatrix.MultiplyInto, Numerics/SparseMatrix.cs:46-73;
  * ResolutionDiagonal.Estimate's A z and TradeOff's G x, Tomography/ResolutionDiagonal.cs:48,

@@ -31,7 +31,7 @@ void or_metric(const or_griddef *g, double *hlon, double *hlat, double *hr)
         double r = OR_R - depth;
         hlat[k] = r * dlat_rad;
         for (int j = 0; j < g->ny; j++) {
-            double c = cos((g->min_lat + j * gd_dlat(g)) * OR_D2R);
+            double c = or_cos((g->min_lat + j * gd_dlat(g)) * OR_D2R);
             double v = r * c * dlon_rad;
             hlon[k * g->ny + j] = v > 1e-9 ? v : 1e-9;
         }
@@ -52,7 +52,7 @@ int or_point_source(const or_griddef *g, const double *slowness, double lon, dou
     or_to_cartesian(lon, lat, dep, src);
     double s_src = or_interpolate(g, slowness, lon, lat, dep);
     double r = OR_R - dep;
-    double h_lon = r * cos(lat * OR_D2R) * (gd_dlon(g) * OR_D2R);
+    double h_lon = r * or_cos(lat * OR_D2R) * (gd_dlon(g) * OR_D2R);
     if (!(h_lon > 1e-9)) h_lon = 1e-9;
     double h_lat = r * (gd_dlat(g) * OR_D2R);
     double h_r = dd;

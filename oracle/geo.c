@@ -5,6 +5,41 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <quadmath.h>
+
+/* Correctly rounded trigonometry: the 113-bit quadmath value (error below 2 ulp of quad) rounded to
+ * the nearest double. It misses the correct rounding only when the exact value lies within 2^-110 of a
+ * midpoint between doubles. */
+double or_sin(double x) { return (double)sinq((__float128)x); }
+double or_cos(double x) { return (double)cosq((__float128)x); }
+double or_asin(double x) { return (double)asinq((__float128)x); }
+double or_atan2(double y, double x) { return (double)atan2q((__float128)y, (__float128)x); }
+
+/* Arrays of either: out[q] = f(a[q]) (func 0 sin, 1 cos, 2 asin) or atan2(a[q], b[q]) (func 3), the
+ * correctly rounded values when correct != 0, else the platform's. */
+void or_trig_array(int func, int correct, int n, const double *a, const double *b, double *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < n; q++) {
+        const double x = a[q], y = b ? b[q] : 0.0;
+        if (!correct) out[q] = or_libm(func, x, y);
+        else if (func == 0) out[q] = or_sin(x);
+        else if (func == 1) out[q] = or_cos(x);
+        else if (func == 2) out[q] = or_asin(x);
+        else out[q] = or_atan2(x, y);
+    }
+}
+
+/* func 0 sin(a), 1 cos(a), 2 asin(a), 3 atan2(a, b) */
+double or_libm(int func, double a, double b)
+{
+    switch (func) {
+    case 0: return sin(a);
+    case 1: return cos(a);
+    case 2: return asin(a);
+    default: return atan2(a, b);
+    }
+}
 
 #define OR_R 6371.0                 /* GeoMath.cs:43 EarthRadiusKm */
 #define OR_D2R (M_PI / 180.0)       /* GeoMath.cs:45 */
@@ -21,10 +56,10 @@ void or_to_cartesian(double lon, double lat, double depth, double out[3])
 {
     double r = OR_R - depth;
     double la = lat * OR_D2R, lo = lon * OR_D2R;
-    double cl = cos(la);
-    out[0] = r * cl * cos(lo);
-    out[1] = r * cl * sin(lo);
-    out[2] = r * sin(la);
+    double cl = or_cos(la);
+    out[0] = r * cl * or_cos(lo);
+    out[1] = r * cl * or_sin(lo);
+    out[2] = r * or_sin(la);
 }
 
 /* GeoMath.cs:61-68 (out = lon, lat, depth) */
@@ -32,8 +67,8 @@ void or_from_cartesian(const double v[3], double out[3])
 {
     double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     if (r <= 0) { out[0] = 0; out[1] = 0; out[2] = OR_R; return; }
-    out[1] = asin(clampd(v[2] / r, -1, 1)) * OR_R2D;
-    out[0] = atan2(v[1], v[0]) * OR_R2D;
+    out[1] = or_asin(clampd(v[2] / r, -1, 1)) * OR_R2D;
+    out[0] = or_atan2(v[1], v[0]) * OR_R2D;
     out[2] = OR_R - r;
 }
 
@@ -41,7 +76,7 @@ void or_from_cartesian(const double v[3], double out[3])
 void or_local_frame(double lon, double lat, double east[3], double north[3], double up[3])
 {
     double la = lat * OR_D2R, lo = lon * OR_D2R;
-    double sl = sin(la), cl = cos(la), so = sin(lo), co = cos(lo);
+    double sl = or_sin(la), cl = or_cos(la), so = or_sin(lo), co = or_cos(lo);
     east[0] = -so; east[1] = co; east[2] = 0;
     north[0] = -sl * co; north[1] = -sl * so; north[2] = cl;
     up[0] = cl * co; up[1] = cl * so; up[2] = sl;
@@ -60,9 +95,9 @@ double or_surface_distance_km(double lon1, double lat1, double lon2, double lat2
 {
     double p1 = lat1 * OR_D2R, p2 = lat2 * OR_D2R;
     double dp = p2 - p1, dl = (lon2 - lon1) * OR_D2R;
-    double a = sin(dp / 2) * sin(dp / 2) + cos(p1) * cos(p2) * sin(dl / 2) * sin(dl / 2);
+    double a = or_sin(dp / 2) * or_sin(dp / 2) + or_cos(p1) * or_cos(p2) * or_sin(dl / 2) * or_sin(dl / 2);
     double s = sqrt(a);
-    return 2 * asin(s < 1 ? s : 1) * OR_R;
+    return 2 * or_asin(s < 1 ? s : 1) * OR_R;
 }
 
 /* SphericalGrid.cs:59-74 GridDefinition.FromSpacing; Math.Round is round-half-even (rint). */
@@ -70,7 +105,7 @@ void or_grid_from_spacing(double min_lon, double max_lon, double min_lat, double
                           double min_depth, double max_depth, double h_km, double v_km, or_griddef *out)
 {
     double mid = 0.5 * (min_lat + max_lat) * OR_D2R;
-    double width = (max_lon - min_lon) * OR_D2R * OR_R * cos(mid);
+    double width = (max_lon - min_lon) * OR_D2R * OR_R * or_cos(mid);
     double height = (max_lat - min_lat) * OR_D2R * OR_R;
     int nx = (int)rint(width / h_km) + 1, ny = (int)rint(height / h_km) + 1;
     int nz = (int)rint((max_depth - min_depth) / v_km) + 1;
@@ -139,7 +174,7 @@ double or_min_spacing_km(const or_griddef *g)
     double rmin = OR_R - (g->min_depth + (g->nz - 1) * ddep(g));
     double cosmin = INFINITY;
     for (int j = 0; j < g->ny; j++) {
-        double c = cos((g->min_lat + j * dlat(g)) * OR_D2R);
+        double c = or_cos((g->min_lat + j * dlat(g)) * OR_D2R);
         if (c < cosmin) cosmin = c;
     }
     double dlon_rad = dlon(g) * OR_D2R, dlat_rad = dlat(g) * OR_D2R;

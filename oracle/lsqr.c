@@ -65,6 +65,43 @@ double or_dot(int n, const double *a, const double *b)
     return sum;
 }
 
+/* LsqrOpenCl.cs:79-94 dot_partial with 64 groups of 256 items: item l of group g sums a[i] b[i] over
+ * i = 256 g + l + k 16384, the group halves its 256 sums in a tree, the host adds the 64 group sums in
+ * order (LsqrOpenCl.cs Dot). */
+double or_dot_device(int n, const double *a, const double *b)
+{
+    double part[64];
+    for (int g = 0; g < 64; g++) {
+        double red[256];
+        for (int l = 0; l < 256; l++) {
+            double s = 0;
+            for (int64_t i = (int64_t)g * 256 + l; i < n; i += 64 * 256) s += a[i] * b[i];
+            red[l] = s;
+        }
+        for (int h = 128; h > 0; h >>= 1)
+            for (int l = 0; l < h; l++) red[l] += red[l + h];
+        part[g] = red[0];
+    }
+    double s = 0;
+    for (int g = 0; g < 64; g++) s += part[g];
+    return s;
+}
+
+/* which dot product or_lsqr uses: 0 SimdVector's (Lsqr.Solve), 1 the reference device's (LsqrOpenCl.Run,
+ * whose norm is the square root of that dot product) */
+static _Thread_local int lsqr_dot_mode = 0;
+void or_lsqr_dot_mode(int mode) { lsqr_dot_mode = mode; }
+static double dot_sel(int n, const double *a, const double *b)
+{
+    return lsqr_dot_mode ? or_dot_device(n, a, b) : or_dot(n, a, b);
+}
+static double norm_sel(int n, const double *a)
+{
+    if (!lsqr_dot_mode) return or_norm(n, a);
+    double d = or_dot_device(n, a, a);
+    return sqrt(d > 0 ? d : 0);
+}
+
 static double max_abs(int n, const double *a)
 {
     double m = 0;
@@ -109,13 +146,13 @@ int or_lsqr(int m, int n, const int64_t *rs, const int *ci, const double *va,
     double *tm = malloc(sizeof(double) * m), *tn = malloc(sizeof(double) * n);
     memcpy(u, b, sizeof(double) * m);
     int stop = 0, iterations = 0;
-    double beta = or_norm(m, u);
+    double beta = norm_sel(m, u);
     if (beta == 0) { stop = 9; goto done; }
     double bnorm = beta;
     stats[1] = bnorm;
     scale(m, u, 1.0 / beta);
     or_spmv(n, trs, tci, tva, u, v, threads);
-    double alpha = or_norm(n, v);
+    double alpha = norm_sel(n, v);
     if (alpha == 0) { stats[2] = bnorm; stop = 10; goto done; }
     scale(n, v, 1.0 / alpha);
     memcpy(w, v, sizeof(double) * n);
@@ -127,13 +164,13 @@ int or_lsqr(int m, int n, const int64_t *rs, const int *ci, const double *va,
         iterations = it + 1;
         or_spmv(m, rs, ci, va, v, tm, threads);
         xmby(m, tm, alpha, u);
-        beta = or_norm(m, u);
+        beta = norm_sel(m, u);
         if (beta > 0) {
             scale(m, u, 1.0 / beta);
             anorm2 += alpha * alpha + beta * beta + damp * damp;
             or_spmv(n, trs, tci, tva, u, tn, threads);
             xmby(n, tn, beta, v);
-            alpha = or_norm(n, v);
+            alpha = norm_sel(n, v);
             if (alpha > 0) scale(n, v, 1.0 / alpha);
         }
         double rhobar1 = sqrt(rhobar * rhobar + damp * damp);
@@ -147,7 +184,7 @@ int or_lsqr(int m, int n, const int64_t *rs, const int *ci, const double *va,
         double phi = cs * phibar;
         phibar = sn * phibar;
         double tau = sn * phi;
-        ddnorm += or_dot(n, w, w) / (rho * rho);
+        ddnorm += dot_sel(n, w, w) / (rho * rho);
         axpy(n, phi / rho, w, x);
         xpby(n, v, -theta / rho, w);
         double delta = sn2 * rho, gambar = -cs2 * rho;

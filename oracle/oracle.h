@@ -9,6 +9,12 @@
  * Conventions (as the reference): grid index (k*Ny + j)*Nx + i, longitude fastest; times in s,
  * distances in km, angles in degrees at the interface; +inf marks an unreached node.
  * Compiled with -ffp-contract=off so products and sums round as the .NET JIT rounds them.
+ *
+ * Trigonometry: the reference calls .NET's Math.Sin / Cos / Asin / Atan2, which return the platform C
+ * library's results, and those differ between platforms in the last bit (glibc 2.35's are not
+ * correctly rounded on 0.07-0.6 % of a regional grid's angles). The oracle uses the correctly rounded
+ * values, the one choice every platform can share: libquadmath's 113-bit sinq / cosq / asinq / atan2q
+ * rounded to double (or_sin, or_cos, or_asin, or_atan2).
  */
 #ifndef TOMOSTAR_ORACLE_H
 #define TOMOSTAR_ORACLE_H
@@ -24,6 +30,15 @@ typedef struct {
     double min_lon, max_lon, min_lat, max_lat, min_depth, max_depth;
     int nx, ny, nz;
 } or_griddef;
+
+/* --- correctly rounded trigonometry (the double nearest to the exact value) -------------------- */
+double or_sin(double x);
+double or_cos(double x);
+double or_asin(double x);
+double or_atan2(double y, double x);
+/* the platform's (glibc) results, for comparisons only: func 0 sin(a), 1 cos(a), 2 asin(a), 3 atan2(a, b) */
+double or_libm(int func, double a, double b);
+void or_trig_array(int func, int correct, int n, const double *a, const double *b, double *out);
 
 /* --- geometry (Geo/GeoMath.cs, Geo/SphericalGrid.cs) --------------------------------------- */
 void or_to_cartesian(double lon, double lat, double depth, double out[3]);
@@ -79,6 +94,10 @@ void or_spmv(int rows, const int64_t *rs, const int *ci, const double *va, const
              int threads);
 double or_dot(int n, const double *a, const double *b);
 double or_norm(int n, const double *a);
+/* LsqrOpenCl's device dot product (64 partials of 256 tree-reduced item sums), and the switch that
+ * makes or_lsqr (on the calling thread) use it and its square root for the norms, as LsqrOpenCl.Run does */
+double or_dot_device(int n, const double *a, const double *b);
+void or_lsqr_dot_mode(int mode);
 int or_lsqr(int m, int n, const int64_t *rs, const int *ci, const double *va,
             const int64_t *trs, const int *tci, const double *tva, const double *b, double damp,
             int max_it, double atol, double btol, double conlim, int threads,

@@ -63,6 +63,7 @@ _G = C.POINTER(GridDef)
 
 def _declare(L):
     d = C.c_double
+    L.or_trig_array.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_void_p, _dp]
     L.or_to_cartesian.argtypes = [d, d, d, _dp]
     L.or_from_cartesian.argtypes = [_dp, _dp]
     L.or_local_frame.argtypes = [d, d, _dp, _dp, _dp]
@@ -89,6 +90,20 @@ def _declare(L):
         _declare_rays(L)
     if hasattr(L, "or_lsqr"):
         _declare_lsqr(L)
+
+
+TRIG = {"sin": 0, "cos": 1, "asin": 2, "atan2": 3}
+
+
+def trig(func: str, a, b=None, correct: bool = True) -> np.ndarray:
+    """sin / cos / asin of a, or atan2(a, b): the correctly rounded values (libquadmath's 113-bit
+    functions rounded to double) or, with correct=False, the platform C library's."""
+    L = lib()
+    a = np.ascontiguousarray(a, np.float64)
+    bb = None if b is None else np.ascontiguousarray(b, np.float64)
+    out = np.empty_like(a)
+    L.or_trig_array(TRIG[func], 1 if correct else 0, a.size, a, None if bb is None else bb.ctypes.data, out)
+    return out
 
 
 def _declare_rays(L):
@@ -120,6 +135,9 @@ def _declare_lsqr(L):
     L.or_lsqr.argtypes = [C.c_int, C.c_int, _lp, _ip, _dp, _lp, _ip, _dp, _dp, d, C.c_int, d, d, d,
                           C.c_int, _dp, _dp, _dp]
     L.or_lsqr.restype = C.c_int
+    L.or_lsqr_dot_mode.argtypes = [C.c_int]
+    L.or_dot_device.argtypes = [C.c_int, _dp, _dp]
+    L.or_dot_device.restype = d
 
 
 # ---------------------------------------------------------------- geometry
@@ -267,8 +285,10 @@ def spmv(rs, ci, va, x, threads=0):
 
 
 def lsqr(m, n, rs, ci, va, b, damp=0.0, max_it=500, atol=1e-8, btol=1e-8, conlim=1e8, threads=0,
-         transpose=None):
-    """Lsqr.Solve (Numerics/Lsqr.cs:34-152). Returns (x, stop code, stats dict, rnorm history)."""
+         transpose=None, device_dots=False):
+    """Lsqr.Solve (Numerics/Lsqr.cs:34-152). Returns (x, stop code, stats dict, rnorm history).
+    device_dots: the dot products and norms of LsqrOpenCl.Run (LsqrOpenCl.cs:79-94: 64 partials of
+    tree-reduced item sums, the norm their square root) instead of SimdVector's."""
     rs = np.ascontiguousarray(rs, np.int64)
     ci = np.ascontiguousarray(ci, np.int32)
     va = np.ascontiguousarray(va, np.float64)
@@ -276,8 +296,13 @@ def lsqr(m, n, rs, ci, va, b, damp=0.0, max_it=500, atol=1e-8, btol=1e-8, conlim
     x = np.zeros(n)
     stats = np.zeros(4)
     hist = np.zeros(max(1, max_it))
-    stop = lib().or_lsqr(m, n, rs, ci, va, trs, tci, tva, np.ascontiguousarray(b, np.float64), damp, max_it,
+    L = lib()
+    L.or_lsqr_dot_mode(1 if device_dots else 0)
+    try:
+        stop = L.or_lsqr(m, n, rs, ci, va, trs, tci, tva, np.ascontiguousarray(b, np.float64), damp, max_it,
                          atol, btol, conlim, threads, x, stats, hist)
+    finally:
+        L.or_lsqr_dot_mode(0)
     it = int(stats[0])
     return x, stop, {"iterations": it, "bnorm": stats[1], "rnorm": stats[2], "acond": stats[3]}, hist[:it].copy()
 

@@ -60,7 +60,7 @@ void or_table_gradient(const or_griddef *g, const float *table, double lon, doub
     double tz = (or_table_time_fractional(g, table, f[0], f[1], f[2] + d) -
                  or_table_time_fractional(g, table, f[0], f[1], f[2] - d)) / (2 * d);
     double r = OR_R - depth;
-    double hx = r * cos(lat * OR_D2R) * (gd_dlon(g) * OR_D2R);
+    double hx = r * or_cos(lat * OR_D2R) * (gd_dlon(g) * OR_D2R);
     double hy = r * (gd_dlat(g) * OR_D2R);
     double hz = gd_ddep(g);
     out[0] = tx / hx;

@@ -58,7 +58,7 @@ static void axis_weights(const or_griddef *g, double vw, int in_nodes, double w[
     double ddep = (g->max_depth - g->min_depth) / (g->nz - 1 > 1 ? g->nz - 1 : 1);
     double r = OR_R - g->min_depth;
     double lat = g->min_lat + (g->ny / 2) * dlat;
-    double hx = r * cos(lat * OR_D2R) * (dlon * OR_D2R);
+    double hx = r * or_cos(lat * OR_D2R) * (dlon * OR_D2R);
     if (!(hx > 1e-9)) hx = 1e-9;
     double hy = r * (dlat * OR_D2R), hz = ddep;
     double h = sqrt(hx * hy);

@@ -115,3 +115,34 @@ def test_planted_mistake_is_caught(oracle, golden):
                     atol=0, btol=0, conlim=0)[0]
     ref = golden[key + "_x"]
     assert np.linalg.norm(x - ref) > 1e-8 * np.linalg.norm(ref)
+
+
+def test_device_dot_product_order(oracle):
+    """or_dot_device restates LsqrOpenCl's dot_partial (LsqrOpenCl.cs:79-94): exact on integers in any
+    order, and on random values equal to a direct emulation of 64 groups x 256 strided item sums, each
+    group halved in a tree, the group sums added in order - and different from SimdVector's order."""
+    L = oracle.lib()
+    rng = np.random.default_rng(5)
+    a = rng.integers(-1000, 1000, 50_000).astype(np.float64)
+    b = rng.integers(-1000, 1000, 50_000).astype(np.float64)
+    assert L.or_dot_device(a.size, a, b) == float(np.dot(a.astype(np.int64), b.astype(np.int64)))
+    x, y = rng.standard_normal(70_001), rng.standard_normal(70_001)
+    part = []
+    for g in range(64):
+        red = [0.0] * 256
+        for lane in range(256):
+            s = 0.0
+            for i in range(g * 256 + lane, x.size, 64 * 256):
+                s += x[i] * y[i]
+            red[lane] = s
+        h = 128
+        while h:
+            for lane in range(h):
+                red[lane] += red[lane + h]
+            h >>= 1
+        part.append(red[0])
+    want = 0.0
+    for p in part:
+        want += p
+    assert L.or_dot_device(x.size, x, y) == want
+    assert L.or_dot(x.size, x, y) != want

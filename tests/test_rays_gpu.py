@@ -46,10 +46,10 @@ def _compare(batch, ref):
             continue
         compared += 1
         nodes, lens, tcalc, ge, gn, gz = rr
-        # the same trilinear arithmetic on the same table: TCalc to the bit; the gradient divides by
-        # r cos(lat) (device cos vs host libm: last-bit differences)
+        # the same trilinear arithmetic on the same table and the same correctly rounded cos(lat):
+        # TCalc and the gradient to the bit
         assert dev.tcalc == tcalc
-        assert np.allclose([dev.ge, dev.gn, dev.gz], [ge, gn, gz], rtol=1e-12, atol=0)
+        assert (dev.ge, dev.gn, dev.gz) == (ge, gn, gz)
         if np.array_equal(nodes, dev.nodes):
             same_cells += 1
             worst_len = max(worst_len, float(np.max(np.abs(dev.length - lens) / np.maximum(lens, 1e-30))))
@@ -92,11 +92,12 @@ def test_rows_match_the_oracle_refined_forward_grid(device, oracle):
 
 
 def test_rows_on_a_shared_grid(device, oracle, iteration_case):
-    """Forward grid = inversion grid (ForwardRefinement = 1): the ray step equals the piece length
-    h, so ceil(len / h) (RayTracing.cs:139) decides 1 or 2 pieces per segment on the last bit of the
-    segment length, which follows the platform's sin/cos/asin/atan2. Cell lists then differ wherever
-    the device's and the host libm's last bits differ; the path lengths and the rows' travel times
-    must still agree to 1e-4."""
+    """Forward grid = inversion grid (ForwardRefinement = 1, the one-iteration case's 61 x 61 x 31
+    grid): the ray step equals the piece length h, so ceil(len / h) (RayTracing.cs:139) decides 1 or 2
+    pieces per segment on the last bit of the segment length, which follows every sin, cos, asin and
+    atan2 along the path. The device and the oracle both round those correctly, so the paths are the
+    same doubles and the cell lists must be identical for >= 99.9 % of rays (the north-star bar);
+    the path lengths and the rows' travel times agree to 1e-4 in any case."""
     from tomostar_amd import rays as Ry
 
     wl, ts, mof = iteration_case
@@ -107,8 +108,10 @@ def test_rows_on_a_shared_grid(device, oracle, iteration_case):
     step = Ry.ray_step(wl.grid, wl.grid)
     ref = oracle.ray_rows_batch(og, list(ts.read_f32()), stations, og, field, events, step)
     compared, same, worst = _compare(batch, ref)
-    print(f"shared grid: {same}/{compared} identical cell lists")
+    print(f"shared grid: {same}/{compared} identical cell lists, worst length error {worst:.2e}")
     assert compared > 0.99 * len(ref)
+    assert same >= 0.999 * compared
+    assert worst <= 1e-4
     s = np.concatenate([wl.slowness_p, wl.slowness_s])
     for r in range(0, len(ref), 37):
         if ref[r] is None:

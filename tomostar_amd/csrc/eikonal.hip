@@ -63,7 +63,7 @@ void grid_metric(const Grid &g, double *hlon, double *hlat, double *hr)
         double r = kEarthRadiusKm - g.depth(k);
         hlat[k] = r * g.dlat_rad;
         for (int j = 0; j < g.ny; j++) {
-            double v = r * cos(g.lat(j) * kDeg2Rad) * g.dlon_rad;
+            double v = r * cr::cos(g.lat(j) * kDeg2Rad) * g.dlon_rad;
             hlon[k * g.ny + j] = v > 1e-9 ? v : 1e-9;
         }
     }
@@ -94,7 +94,7 @@ int point_source(const Grid &g, const double *slowness, double lon, double lat, 
     Vec3 src = to_cartesian(lon, lat, dep);
     double s_src = interpolate(g, slowness, lon, lat, dep);
     double r = kEarthRadiusKm - dep;
-    double h_lon = r * cos(lat * kDeg2Rad) * g.dlon_rad;
+    double h_lon = r * cr::cos(lat * kDeg2Rad) * g.dlon_rad;
     if (!(h_lon > 1e-9)) h_lon = 1e-9;
     double h_lat = r * g.dlat_rad, h_r = g.ddepth;
     double radius = radius_cells * std::max(h_lon, std::max(h_lat, h_r));

@@ -2,7 +2,9 @@
 //
 // GeoMath (Geo/GeoMath.cs:40-151) and SphericalGrid (Geo/SphericalGrid.cs:84-235): a sphere of radius
 // 6371 km, geographic latitude used as geocentric, nodes ON the bounds, index (k*Ny + j)*Nx + i.
-// Every formula keeps the reference's operation order so positions round the same way.
+// Every formula keeps the reference's operation order so positions round the same way, and every
+// sine, cosine, arcsine and arctangent is the correctly rounded one (crmath.hpp), so positions are the
+// same doubles on the device as on any host that rounds its trigonometry correctly.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,6 +12,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "crmath.hpp"
 #include "tomostar_hip.h"
 
 namespace ts {
@@ -34,8 +37,10 @@ __host__ __device__ inline Vec3 to_cartesian(double lon_deg, double lat_deg, dou
     double r = kEarthRadiusKm - depth_km;
     double lat = lat_deg * kDeg2Rad;
     double lon = lon_deg * kDeg2Rad;
-    double cl = cos(lat);
-    return {r * cl * cos(lon), r * cl * sin(lon), r * sin(lat)};
+    double sl, cl, so, co;
+    cr::sincos(lat, sl, cl);
+    cr::sincos(lon, so, co);
+    return {r * cl * co, r * cl * so, r * sl};
 }
 
 struct GeoPoint {
@@ -47,8 +52,8 @@ __host__ __device__ inline GeoPoint from_cartesian(Vec3 v)
 {
     double r = vlen(v);
     if (r <= 0) return {0, 0, kEarthRadiusKm};
-    double lat = asin(dclamp(v.z / r, -1, 1)) * kRad2Deg;
-    double lon = atan2(v.y, v.x) * kRad2Deg;
+    double lat = cr::asin(dclamp(v.z / r, -1, 1)) * kRad2Deg;
+    double lon = cr::atan2(v.y, v.x) * kRad2Deg;
     return {lon, lat, kEarthRadiusKm - r};
 }
 
@@ -152,7 +157,7 @@ inline double min_spacing_km(const Grid &g)
     double rmin = kEarthRadiusKm - g.depth_last;
     double cosmin = INFINITY;
     for (int j = 0; j < g.ny; j++) {
-        double c = cos(g.lat(j) * kDeg2Rad);
+        double c = cr::cos(g.lat(j) * kDeg2Rad);
         if (c < cosmin) cosmin = c;
     }
     double a = rmin * g.dlat_rad, b = rmin * cosmin * g.dlon_rad;

@@ -80,7 +80,7 @@ __device__ inline void table_gradient(const Grid &g, const float *T, double lon,
     const double ty = (table_time_frac(g, T, fx, fy + d, fz) - table_time_frac(g, T, fx, fy - d, fz)) / (2 * d);
     const double tz = (table_time_frac(g, T, fx, fy, fz + d) - table_time_frac(g, T, fx, fy, fz - d)) / (2 * d);
     const double r = kEarthRadiusKm - depth;
-    ge = tx / (r * cos(lat * kDeg2Rad) * g.dlon_rad);
+    ge = tx / (r * cr::cos(lat * kDeg2Rad) * g.dlon_rad);
     gn = ty / (r * g.dlat_rad);
     gz = tz / g.ddepth;
 }
@@ -133,7 +133,9 @@ __global__ __launch_bounds__(64) void ray_trace(Grid g, const float *__restrict_
         if (!__builtin_isfinite(e1) || !__builtin_isfinite(e2) || !__builtin_isfinite(e3)) { status = -1; break; }
         // GeoMath.LocalFrame (GeoMath.cs:71-80)
         const double la = gp.lat * kDeg2Rad, lo = gp.lon * kDeg2Rad;
-        const double sl = sin(la), cl = cos(la), so = sin(lo), co = cos(lo);
+        double sl, cl, so, co;
+        cr::sincos(la, sl, cl);
+        cr::sincos(lo, so, co);
         const Vec3 east{-so, co, 0}, north{-sl * co, -sl * so, cl}, up{cl * co, cl * so, sl};
         const Vec3 grad = vadd(vadd(vscale(east, e1), vscale(north, e2)), vscale(up, -e3));
         const double norm = vlen(grad);

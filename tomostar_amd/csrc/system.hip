@@ -457,7 +457,7 @@ void axis_weights(const Grid &g, double vertical, bool in_nodes, double w[3])
     if (in_nodes) { w[0] = 1; w[1] = 1; w[2] = vertical; return; }
     const double r = kEarthRadiusKm - g.min_depth;
     const double lat = g.min_lat + (g.ny / 2) * g.dlat;
-    const double hx = std::max(1e-9, r * std::cos(lat * kDeg2Rad) * (g.dlon * kDeg2Rad));
+    const double hx = std::max(1e-9, r * cr::cos(lat * kDeg2Rad) * (g.dlon * kDeg2Rad));
     const double hy = r * (g.dlat * kDeg2Rad), hz = g.ddepth;
     const double h = std::sqrt(hx * hy);
     w[0] = std::pow(h / hx, 2);
@@ -936,6 +936,28 @@ int ts_matrix_read(ts_matrix *a, int32_t *row_start, int32_t *column, double *va
                 if (values) values[at] = va[q];
             }
             if (row_start) row_start[o + 1] = (int32_t)at;
+        }
+        return TS_OK;
+    });
+}
+
+int ts_matrix_stored_order(ts_matrix *a, int32_t *row_perm, int32_t *col_perm)
+{
+    if (!a) return TS_EINVAL;
+    return guarded(nullptr, 0, [&]() -> int {
+        TS_HIP(hipSetDevice(a->ctx->device));
+        std::lock_guard<std::mutex> lk(a->gate);
+        if (row_perm) {
+            if (a->row_perm.p && a->rows > 0)
+                TS_HIP(hipMemcpy(row_perm, a->row_perm.p, (size_t)a->rows * 4, hipMemcpyDeviceToHost));
+            else
+                for (int r = 0; r < a->rows; r++) row_perm[r] = r;
+        }
+        if (col_perm) {
+            if (a->col_perm.p && a->cols > 0)
+                TS_HIP(hipMemcpy(col_perm, a->col_perm.p, (size_t)a->cols * 4, hipMemcpyDeviceToHost));
+            else
+                for (int c = 0; c < a->cols; c++) col_perm[c] = c;
         }
         return TS_OK;
     });

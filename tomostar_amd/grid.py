@@ -10,10 +10,31 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from . import _lib
 from ._lib import TsGrid
 
 EARTH_RADIUS_KM = 6371.0  # Geo/GeoMath.cs:43
 DEG2RAD = math.pi / 180.0
+
+
+_TRIG = {"sin": 0, "cos": 1, "asin": 2, "atan2": 3}
+
+
+def trig(func: str, a, b=None) -> np.ndarray:
+    """The library's correctly rounded sin / cos / asin of a, or atan2(a, b), evaluated on the host
+    (ts_crmath_eval): the trigonometry every position of the library is computed with."""
+    import ctypes as C
+
+    L = _lib.load()
+    if not getattr(L, "_crmath_declared", False):
+        L.ts_crmath_eval.restype = C.c_int
+        L.ts_crmath_eval.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L._crmath_declared = True
+    a = np.ascontiguousarray(a, np.float64)
+    bb = None if b is None else np.ascontiguousarray(b, np.float64)
+    out = np.empty_like(a)
+    _lib.check(L.ts_crmath_eval(None, _TRIG[func], a.size, _lib.ptr(a), _lib.ptr(bb), _lib.ptr(out)))
+    return out
 
 
 @dataclass(frozen=True)
@@ -87,7 +108,7 @@ class GridDefinition:
     def min_spacing_km(self) -> float:
         """SphericalGrid.MinSpacingKm (SphericalGrid.cs:159-164)."""
         rmin = EARTH_RADIUS_KM - self.depth()[-1]
-        cosmin = float(np.min(np.cos(self.lat() * DEG2RAD)))
+        cosmin = float(np.min(trig("cos", self.lat() * DEG2RAD)))
         return min(self.ddepth, min(rmin * (self.dlat * DEG2RAD), rmin * cosmin * (self.dlon * DEG2RAD)))
 
     def fractional(self, lon, lat, depth):

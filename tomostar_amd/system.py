@@ -62,6 +62,18 @@ class System:
         check(_declare().ts_matrix_read(m.handle, ptr(rs), ptr(ci), ptr(va), None))
         return rs, ci[:m.nnz], va[:m.nnz]
 
+    def stored_order(self):
+        """(row_perm, col_perm): stored row r is given row row_perm[r], stored column c is given column
+        col_perm[c] (ts_matrix_stored_order)."""
+        m = self.matrix
+        rp = np.zeros(max(1, m.rows), np.int32)
+        cp = np.zeros(max(1, m.cols), np.int32)
+        L = _declare()
+        L.ts_matrix_stored_order.restype = C.c_int
+        L.ts_matrix_stored_order.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        check(L.ts_matrix_stored_order(m.handle, ptr(rp), ptr(cp)))
+        return rp[:m.rows], cp[:m.cols]
+
     def read_transpose(self):
         m = self.matrix
         rs = np.zeros(m.cols + 1, np.int32)
