@@ -44,9 +44,7 @@ constexpr int kLanes = 64;
 // FP64 form over every tenth field): 4 ulps (5, 1.089 s, 4.571e-6), 256 (4, 1.068 s, 4.571e-6), 4096
 // (3, 1.029 s, 4.571e-6), 16384 (3, 1.031 s, 4.571e-6), 65536 (3, 1.014 s, 2.2e-5) - the second-order
 // phase takes 6 rounds in each case; from 65536 ulps on its result moves.
-#ifndef TS_FP32_SETTLE_ULPS
-#define TS_FP32_SETTLE_ULPS 4096
-#endif
+constexpr float kFp32SettleUlps = 4096;  // TOMOSTAR_FP32_SETTLE_ULPS overrides
 
 inline int sweep_round_cap(const Grid &g)  // SphericalEikonal.cs:335
 {
@@ -230,6 +228,7 @@ struct FastArgs {
     unsigned long long *evals;    // [1024] spread counters of executed node updates
     const uint32_t *planes;       // nodes of the plane family, packed i' | j' << 11 | k' << 22
     double wake_tol;              // a drop wakes the node's readers when larger than this (0: any drop)
+    float settle_ulps;            // FP32 times: a node moved when it dropped by more than this many ulps
     int nx, ny, nz, N;
 };
 
@@ -245,27 +244,39 @@ __device__ inline T at_off(const T *base, uint32_t byte_off)
 // memory once; read it first, so a clean node costs one load; or know it is set (checked beforehand).
 enum FlagRead { kFlagWithLoads = 0, kFlagFirst = 1, kFlagKnown = 2 };
 
-// Grid node of packed plane entry pk (i' | j' << 11 | k' << 22 in the ordering's mirrored coordinates).
+// Grid node of packed plane entry pk (i' | j' << 11 | k' << 22 in the ordering's mirrored coordinates),
+// and its unmirrored coordinates.
+template <typename R>
+__device__ inline int plane_node(const FastArgs<R> &A, int dir, uint32_t pk, int &i, int &j, int &k)
+{
+    const int ip = pk & 2047, jp = (pk >> 11) & 2047, kp = pk >> 22;
+    i = (dir & 1) ? A.nx - 1 - ip : ip;
+    j = (dir & 2) ? A.ny - 1 - jp : jp;
+    k = (dir & 4) ? A.nz - 1 - kp : kp;
+    return (k * A.ny + j) * A.nx + i;
+}
 template <typename R>
 __device__ inline int plane_node(const FastArgs<R> &A, int dir, uint32_t pk)
 {
-    const int ip = pk & 2047, jp = (pk >> 11) & 2047, kp = pk >> 22;
-    const int i = (dir & 1) ? A.nx - 1 - ip : ip;
-    const int j = (dir & 2) ? A.ny - 1 - jp : jp;
-    const int k = (dir & 4) ? A.nz - 1 - kp : kp;
-    return (k * A.ny + j) * A.nx + i;
+    int i, j, k;
+    return plane_node(A, dir, pk, i, j, k);
 }
 
 // The update of plane node pk for the 64 fields of group g (one wavefront, one lane per field). No
 // node of a plane reads another node of the same plane (neighbours lie one or two planes away), so
 // the order of the loads does not change what they read.
-template <typename R, bool kCount, bool kOff32, int kFlag>
+// kLean: the mixed form's arithmetic (godunov_lean) instead of the reference's (FP64: Relax's, bit for
+// bit; FP32: the correctly rounded offset form).
+template <typename R, bool kCount, bool kOff32, int kFlag, bool kLean>
 __device__ inline void update_node(const FastArgs<R> &A, int dir, uint32_t pk, int g, uint64_t act, int lane,
                                    unsigned counter)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
-    const int node = plane_node(A, dir, pk);
-    const int i = node % nx, j = (node / nx) % ny, k = node / nxy;
+    // every lane of the wave updates the same node (one field per lane): the node's coordinates and
+    // addresses are wave-uniform, held in scalar registers
+    pk = __builtin_amdgcn_readfirstlane(pk);
+    int i, j, k;
+    const int node = plane_node(A, dir, pk, i, j, k);
     const size_t gn = (size_t)g * A.N;
     uint8_t *flag = A.dirty + gn + node;
     uint8_t fl = 1;
@@ -312,13 +323,15 @@ __device__ inline void update_node(const FastArgs<R> &A, int dir, uint32_t pk, i
             t2[ax] = (second && isfinite(first[ax])) ? beyond : (R)INFINITY;
         }
         R next;
-        if constexpr (sizeof(R) == 4) next = (s > 0.0f) ? godunov_offset(s, first, t2, wa) : (R)INFINITY;
+        if constexpr (kLean && sizeof(R) == 4) next = (s > 0.0f) ? godunov_lean32(s, first, t2, wa) : (R)INFINITY;
+        else if constexpr (kLean) next = (s > 0.0) ? godunov_lean64(s, first, t2, wa) : (R)INFINITY;
+        else if constexpr (sizeof(R) == 4) next = (s > 0.0f) ? godunov_offset(s, first, t2, wa) : (R)INFINITY;
         else next = (s > 0.0) ? godunov_abs_w<double>(s, first, t2, wa) : (R)INFINITY;
         if (next < cur) {
             Tg[(size_t)node * kLanes + lane] = next;
             dropped = !isfinite(cur) || (double)(cur - next) > A.wake_tol;
-            if constexpr (sizeof(R) == 4 && TS_FP32_SETTLE_ULPS > 0)
-                moved = !isfinite(cur) || (cur - next) > fmaxf((float)kSettleTol, TS_FP32_SETTLE_ULPS * 1.1920929e-7f * cur);
+            if constexpr (sizeof(R) == 4)
+                moved = !isfinite(cur) || (cur - next) > fmaxf((float)kSettleTol, A.settle_ulps * 1.1920929e-7f * cur);
             else
                 moved = !isfinite(cur) || (double)(cur - next) > kSettleTol;
         }
@@ -349,7 +362,7 @@ __device__ inline void update_node(const FastArgs<R> &A, int dir, uint32_t pk, i
 // updates plane nodes a, a + W, a + 2W, ... (W the waves of a group's row of blocks), so a launch
 // is a few resident waves per SIMD rather than one short-lived workgroup per node. kFlag: the dirty
 // flag read first (rounds where most nodes are clean) or with every other load (where most are not).
-template <typename R, int WPB, bool kCount, bool kOff32, int kFlag>
+template <typename R, int WPB, bool kCount, bool kOff32, int kFlag, bool kLean>
 __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
@@ -363,13 +376,13 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int di
     const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     const int stride = 8 * per_xcd * WPB;
     for (int a = lb * WPB + (threadIdx.x >> 6); a < cnt; a += stride)
-        update_node<R, kCount, kOff32, kFlag>(A, dir, A.planes[off + a], g, act, lane, blockIdx.x);
+        update_node<R, kCount, kOff32, kFlag, kLean>(A, dir, A.planes[off + a], g, act, lane, blockIdx.x);
 }
 
 // Rounds in which most nodes are clean: each wavefront reads the dirty flags of M consecutive plane
 // nodes at once (lanes 0 .. M-1) and then updates its dirty ones in plane order, so a clean stretch of
 // the plane costs one load per M nodes and M times fewer waves are launched.
-template <typename R, int WPB, int M, bool kCount, bool kOff32>
+template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean>
 __global__ __launch_bounds__(64 * WPB) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
@@ -388,7 +401,7 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane_sparse(FastArgs<R> A,
     }
     for (uint64_t todo = __ballot(dirty); todo; todo &= todo - 1) {
         const uint32_t p = __shfl(pk, __builtin_ctzll(todo));
-        update_node<R, kCount, kOff32, kFlagKnown>(A, dir, p, g, act, lane, blockIdx.x);
+        update_node<R, kCount, kOff32, kFlagKnown, kLean>(A, dir, p, g, act, lane, blockIdx.x);
     }
 }
 
@@ -564,14 +577,22 @@ struct ts_tableset {
     dbuf<int32_t> model_dev, state, rounds_dev, status;
     dbuf<unsigned long long> evals;
     double wake_tol[2] = {0, 0};          // FastArgs::wake_tol of the running solve's FP32 / FP64 sweeps
+    bool lean[2] = {false, false};        // the running solve's FP32 / FP64 sweeps use the mixed form's arithmetic
+    float settle_ulps = 4096;             // the running solve's FP32 settle threshold (ulps of the time)
+    // launch-shape knobs, read once when the table set is created (graphs captured with them stay valid)
+    int knob_group_chunk = 0, knob_plane_blocks = 0;
     hipGraphExec_t graphs[2][2][3][2] = {};  // [FP64][counting][flag reads][wake_tol > 0]
     double graph_wake_tol[2][2][3][2] = {};  // the wake_tol each graph was captured with
+    bool graph_lean[2][2][3][2] = {};        // and its arithmetic
+    float graph_ulps[2][2][3][2] = {};       // and its FP32 settle threshold
     // chunks of groups sweeping side by side, each on its own stream with its own graph of plane launches
     static constexpr int kMaxStreams = 8;
     hipStream_t gstream[kMaxStreams] = {};
     hipEvent_t gfork = nullptr, gjoin[kMaxStreams] = {};
     hipGraphExec_t cgraphs[2][2][3][2][kMaxStreams] = {};
     double cgraph_wake_tol[2][2][3][2][kMaxStreams] = {};
+    bool cgraph_lean[2][2][3][2][kMaxStreams] = {};
+    float cgraph_ulps[2][2][3][2][kMaxStreams] = {};
     int cgraph_groups[2][2][3][2][kMaxStreams][2] = {};  // the groups [first, first + count) each swept
     std::mutex gate;
     ~ts_tableset()
@@ -751,6 +772,7 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
     A.evals = t.evals.p + (sizeof(R) == 8 ? 1024 : 0);  // FP32 and FP64 updates counted apart
     A.planes = t.planes.p;
     A.wake_tol = t.wake_tol[sizeof(R) == 8];
+    A.settle_ulps = t.settle_ulps;
     A.nx = t.grid.nx; A.ny = t.grid.ny; A.nz = t.grid.nz; A.N = t.N;
     return A;
 }
@@ -765,23 +787,24 @@ constexpr int kSparseNodes = TS_SPARSE_NODES;
 // Cap on the blocks of a plane launch over all its groups (TOMOSTAR_PLANE_BLOCKS): waves stride over
 // the plane beyond it. Measured on the 400-field case: fewer, longer-lived waves are slower (the
 // sweep is bound by the memory requests in flight), so by default every plane node gets its own wave.
-int plane_blocks(const ts_tableset &t)
-{
-    (void)t;
-    if (const char *e = getenv("TOMOSTAR_PLANE_BLOCKS")) return std::max(1, atoi(e));
-    return 1 << 30;
-}
+int plane_blocks(const ts_tableset &t) { return t.knob_plane_blocks > 0 ? t.knob_plane_blocks : 1 << 30; }
 
 // Groups swept together: TOMOSTAR_GROUP_CHUNK when set, else all of them.
 int group_chunk(const ts_tableset &t, size_t width)
 {
     (void)width;
-    if (const char *e = getenv("TOMOSTAR_GROUP_CHUNK")) return std::max(1, atoi(e));
-    return (t.nfields + kLanes - 1) / kLanes;
+    return t.knob_group_chunk > 0 ? t.knob_group_chunk : (t.nfields + kLanes - 1) / kLanes;
+}
+
+// The launch-shape knobs of a new table set (TOMOSTAR_GROUP_CHUNK, TOMOSTAR_PLANE_BLOCKS), read once.
+void read_launch_knobs(ts_tableset &t)
+{
+    if (const char *e = getenv("TOMOSTAR_GROUP_CHUNK")) t.knob_group_chunk = std::max(1, atoi(e));
+    if (const char *e = getenv("TOMOSTAR_PLANE_BLOCKS")) t.knob_plane_blocks = std::max(1, atoi(e));
 }
 
 // The plane launches of groups [g0, g0 + gc): all eight orderings, every plane.
-template <typename R, bool kCount, int kOrder>
+template <typename R, bool kCount, int kOrder, bool kLean>
 void launch_planes(ts_tableset &t, R *T, hipStream_t st, int g0, int gc)
 {
     FastArgs<R> A = fast_args<R>(t, T);
@@ -796,15 +819,15 @@ void launch_planes(ts_tableset &t, R *T, hipStream_t st, int g0, int gc)
                 const unsigned nb = blocks_for(cnt, kWPB * kSparseNodes);
                 dim3 grid((nb + 7) / 8 * 8, gc);
                 if (off32)
-                    eik_fast_plane_sparse<R, kWPB, kSparseNodes, kCount, true><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                    eik_fast_plane_sparse<R, kWPB, kSparseNodes, kCount, true, kLean><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
                 else
-                    eik_fast_plane_sparse<R, kWPB, kSparseNodes, kCount, false><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                    eik_fast_plane_sparse<R, kWPB, kSparseNodes, kCount, false, kLean><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
             } else {
                 // a multiple of 8 blocks: each XCD gets the same number of slices
                 const unsigned nb = std::min<unsigned>(blocks_for(cnt, kWPB), std::max(1, plane_blocks(t) / gc));
                 dim3 grid((nb + 7) / 8 * 8, gc);
-                if (off32) eik_fast_plane<R, kWPB, kCount, true, kOrder><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
-                else eik_fast_plane<R, kWPB, kCount, false, kOrder><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                if (off32) eik_fast_plane<R, kWPB, kCount, true, kOrder, kLean><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                else eik_fast_plane<R, kWPB, kCount, false, kOrder, kLean><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
             }
         }
 }
@@ -819,7 +842,7 @@ void round_end(ts_tableset &t, hipStream_t st, int hold)
     eik_fast_reset_dirty<<<rg, 256, 0, st>>>(t.dirty.p, t.reset.p, t.N);
 }
 
-template <typename R, bool kCount, int kOrder>
+template <typename R, bool kCount, int kOrder, bool kLean>
 void launch_round(ts_tableset &t, R *T, hipStream_t st, int hold)
 {
     const int groups = (t.nfields + kLanes - 1) / kLanes;
@@ -829,22 +852,29 @@ void launch_round(ts_tableset &t, R *T, hipStream_t st, int hold)
     const int chunk = std::max(1, std::min(groups, group_chunk(t, sizeof(R))));
     t.plane_launches_per_round = ((groups + chunk - 1) / chunk) * 8 * P;
     for (int g0 = 0; g0 < groups; g0 += chunk)
-        launch_planes<R, kCount, kOrder>(t, T, st, g0, std::min(chunk, groups - g0));
+        launch_planes<R, kCount, kOrder, kLean>(t, T, st, g0, std::min(chunk, groups - g0));
     round_end(t, st, hold);
+}
+
+template <typename R, bool kLean>
+void launch_planes_lean(ts_tableset &t, R *T, hipStream_t st, bool count, int order, int g0, int gc)
+{
+    if (count) {
+        if (order == kFlagWithLoads) launch_planes<R, true, kFlagWithLoads, kLean>(t, T, st, g0, gc);
+        else if (order == kFlagFirst) launch_planes<R, true, kFlagFirst, kLean>(t, T, st, g0, gc);
+        else launch_planes<R, true, kSparse, kLean>(t, T, st, g0, gc);
+    } else {
+        if (order == kFlagWithLoads) launch_planes<R, false, kFlagWithLoads, kLean>(t, T, st, g0, gc);
+        else if (order == kFlagFirst) launch_planes<R, false, kFlagFirst, kLean>(t, T, st, g0, gc);
+        else launch_planes<R, false, kSparse, kLean>(t, T, st, g0, gc);
+    }
 }
 
 template <typename R>
 void launch_planes_as(ts_tableset &t, R *T, hipStream_t st, bool count, int order, int g0, int gc)
 {
-    if (count) {
-        if (order == kFlagWithLoads) launch_planes<R, true, kFlagWithLoads>(t, T, st, g0, gc);
-        else if (order == kFlagFirst) launch_planes<R, true, kFlagFirst>(t, T, st, g0, gc);
-        else launch_planes<R, true, kSparse>(t, T, st, g0, gc);
-    } else {
-        if (order == kFlagWithLoads) launch_planes<R, false, kFlagWithLoads>(t, T, st, g0, gc);
-        else if (order == kFlagFirst) launch_planes<R, false, kFlagFirst>(t, T, st, g0, gc);
-        else launch_planes<R, false, kSparse>(t, T, st, g0, gc);
-    }
+    if (t.lean[sizeof(R) == 8]) launch_planes_lean<R, true>(t, T, st, count, order, g0, gc);
+    else launch_planes_lean<R, false>(t, T, st, count, order, g0, gc);
 }
 
 // Streams the groups sweep on side by side when rounds run as graphs (TOMOSTAR_GROUP_STREAMS overrides;
@@ -861,19 +891,25 @@ int group_streams(int groups, bool graphs)
     return std::max(0, std::min({n, groups, (int)ts_tableset::kMaxStreams}));
 }
 
-template <typename R, bool kCount>
+template <typename R, bool kCount, bool kLean>
 void launch_round_order(ts_tableset &t, R *T, hipStream_t st, int hold, int order)
 {
-    if (order == kFlagWithLoads) launch_round<R, kCount, kFlagWithLoads>(t, T, st, hold);
-    else if (order == kFlagFirst) launch_round<R, kCount, kFlagFirst>(t, T, st, hold);
-    else launch_round<R, kCount, kSparse>(t, T, st, hold);
+    if (order == kFlagWithLoads) launch_round<R, kCount, kFlagWithLoads, kLean>(t, T, st, hold);
+    else if (order == kFlagFirst) launch_round<R, kCount, kFlagFirst, kLean>(t, T, st, hold);
+    else launch_round<R, kCount, kSparse, kLean>(t, T, st, hold);
 }
 
 template <typename R>
 void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count, int order)
 {
-    if (count) launch_round_order<R, true>(t, T, st, hold, order);
-    else launch_round_order<R, false>(t, T, st, hold, order);
+    const bool lean = t.lean[sizeof(R) == 8];
+    if (count) {
+        if (lean) launch_round_order<R, true, true>(t, T, st, hold, order);
+        else launch_round_order<R, true, false>(t, T, st, hold, order);
+    } else {
+        if (lean) launch_round_order<R, false, true>(t, T, st, hold, order);
+        else launch_round_order<R, false, false>(t, T, st, hold, order);
+    }
 }
 
 // How round `round` of a run reads the dirty flags:
@@ -913,12 +949,17 @@ hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count, int order)
     const int woken = t.wake_tol[w] > 0;
     hipGraphExec_t &ge = t.graphs[w][count][order][woken];
     double &captured = t.graph_wake_tol[w][count][order][woken];
-    if (ge && captured != t.wake_tol[w]) {  // captured with another tolerance
+    bool &captured_lean = t.graph_lean[w][count][order][woken];
+    float &captured_ulps = t.graph_ulps[w][count][order][woken];
+    // captured with another tolerance, arithmetic or settle threshold (kernel arguments are captured)
+    if (ge && (captured != t.wake_tol[w] || captured_lean != t.lean[w] || captured_ulps != t.settle_ulps)) {
         TS_HIP(hipGraphExecDestroy(ge));
         ge = nullptr;
     }
     if (ge) return ge;
     captured = t.wake_tol[w];
+    captured_lean = t.lean[w];
+    captured_ulps = t.settle_ulps;
     hipStream_t st = t.ctx->stream;
     hipGraph_t gr;
     TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -939,14 +980,20 @@ hipGraphExec_t chunk_graph(ts_tableset &t, R *T, hipStream_t st, bool count, int
     const int woken = t.wake_tol[w] > 0;
     hipGraphExec_t &ge = t.cgraphs[w][count][order][woken][c];
     double &captured = t.cgraph_wake_tol[w][count][order][woken][c];
+    bool &captured_lean = t.cgraph_lean[w][count][order][woken][c];
+    float &captured_ulps = t.cgraph_ulps[w][count][order][woken][c];
     int *range = t.cgraph_groups[w][count][order][woken][c];
-    // captured with another tolerance, or for other groups (the stream count changed between solves)
-    if (ge && (captured != t.wake_tol[w] || range[0] != g0 || range[1] != gc)) {
+    // captured with another tolerance, arithmetic or settle threshold, or for other groups (the stream
+    // count changed between solves)
+    if (ge && (captured != t.wake_tol[w] || captured_lean != t.lean[w] || captured_ulps != t.settle_ulps ||
+               range[0] != g0 || range[1] != gc)) {
         TS_HIP(hipGraphExecDestroy(ge));
         ge = nullptr;
     }
     if (ge) return ge;
     captured = t.wake_tol[w];
+    captured_lean = t.lean[w];
+    captured_ulps = t.settle_ulps;
     range[0] = g0;
     range[1] = gc;
     hipGraph_t gr;
@@ -1117,6 +1164,14 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     // mixed form's first-order phase; its second-order phase only on drops above mixed_wake_tol().
     t.wake_tol[0] = 0.0;
     t.wake_tol[1] = mixed ? mixed_wake_tol() : 0.0;
+    // The mixed form's second-order phase uses godunov_lean64 (TOMOSTAR_MIXED_EXACT=1: Relax's arithmetic);
+    // its first-order phase the correctly rounded FP32 offset form (TOMOSTAR_LEAN_FIRST=1: godunov_lean32).
+    // The FP64 form keeps Relax's arithmetic.
+    const bool exact = getenv("TOMOSTAR_MIXED_EXACT") && atoi(getenv("TOMOSTAR_MIXED_EXACT")) != 0;
+    const bool lean_first = getenv("TOMOSTAR_LEAN_FIRST") && atoi(getenv("TOMOSTAR_LEAN_FIRST")) != 0;
+    t.lean[0] = mixed && lean_first;
+    t.lean[1] = mixed && !exact;
+    t.settle_ulps = getenv("TOMOSTAR_FP32_SETTLE_ULPS") ? (float)atof(getenv("TOMOSTAR_FP32_SETTLE_ULPS")) : kFp32SettleUlps;
     if (mixed) {
         run_rounds<float>(t, t.T32.p, count, cancel, h_status, false, true);
         eik_mixed_widen<<<blocks_for(nodes_total * kLanes, 256), 256, 0, st>>>(t.T32.p, t.T64.p, nodes_total * kLanes);
@@ -1298,6 +1353,7 @@ int tables_create_core(ts_context *ctx, const ts_grid *grid, int nmodels, const 
             TS_HIP(hipMemcpy(t->wlat32.p, wt32.data(), wt32.size() * 4, hipMemcpyHostToDevice));
         }
         build_planes(*t);
+        read_launch_knobs(*t);
         t->rounds.assign(2 * (size_t)nfields, 0);
         *out = t.release();
         return TS_OK;

@@ -181,4 +181,114 @@ __device__ inline float godunov_offset(float s, const float first[3], const floa
     return found ? r + cand : inf;
 }
 
+// The mixed form's updates (no bit-for-bit obligation: the mixed form is held to 1e-4 of Relax). The
+// terms of the reference's quadratic (the same selection as godunov_abs: the largest number of terms
+// whose root is causal, SphericalEikonal.cs:365-395) are formed as offsets from r, the smallest upwind
+// neighbour, and all three candidates are formed without branches (the 64 lanes of a wave are 64 fields
+// that choose differently); the result is r + the chosen offset.
+//
+// godunov_lean64 (the second-order phase on FP64 times, the default): FP64 throughout, with the
+// hardware's reciprocal and reciprocal square root refined by two Newton steps each (relative error
+// ~1e-16) instead of the correctly rounded division and square root sequences. FP32 arithmetic here
+// would add ~1e-8 s of noise to near-degenerate quadratics, enough to keep the phase's 1e-7 s settle
+// test from passing (13 rounds instead of 6 on the 400-field case).
+// godunov_lean32 (the first-order phase on FP32 times; off by default, TOMOSTAR_LEAN_FIRST=1): the
+// quadratic in FP32 with the hardware's square root, reciprocal square root and reciprocal (1 ulp).
+// It equals godunov_offset to 1 ulp (2e7 random first-order inputs), and the phase runs 17 % faster,
+// but Relax's second-order phase has more than one fixed point: started from first-order times that
+// differ by an ulp here and there, it settles elsewhere on large regions (400 fields of 201 x 201 x 101:
+// largest distance to Relax 5.7e-5 against 4.6e-6, 9e4 nodes of one field moved by more than 1e-6 s
+// against 208), so the first-order phase keeps the correctly rounded offset form.
+__device__ inline float godunov_lean32(float s, const float first[3], const float t2[3], const float wa[3][2])
+{
+    const float r = fminf(first[0], fminf(first[1], first[2]));
+    const float inf = INFINITY;
+    if (!(r < inf)) return inf;
+    float te[3], we[3], tn[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const bool on = first[a] < inf;
+        const float of = on ? first[a] - r : inf;
+        const bool so = on && t2[a] <= first[a];
+        te[a] = so ? of + (first[a] - t2[a]) * (1.0f / 3.0f) : of;
+        we[a] = so ? wa[a][1] : wa[a][0];
+        tn[a] = of;
+    }
+#define TS_SWAP(x, y) { auto t_ = x; x = y; y = t_; }
+    if (te[0] > te[1]) { TS_SWAP(te[0], te[1]) TS_SWAP(we[0], we[1]) TS_SWAP(tn[0], tn[1]) }
+    if (te[1] > te[2]) {
+        TS_SWAP(te[1], te[2]) TS_SWAP(we[1], we[2]) TS_SWAP(tn[1], tn[2])
+        if (te[0] > te[1]) { TS_SWAP(te[0], te[1]) TS_SWAP(we[0], we[1]) TS_SWAP(tn[0], tn[1]) }
+    }
+    const float s2 = s * s;
+    const float e1 = te[1] - te[0], e2 = te[2] - te[0];
+    const float root1 = te[0] + s * __builtin_amdgcn_rsqf(we[0]);
+    const float a2 = we[0] + we[1], b2 = we[1] * e1;
+    const float disc2 = a2 * s2 - we[0] * we[1] * e1 * e1;
+    const float root2 = te[0] + (b2 + __builtin_amdgcn_sqrtf(disc2)) * __builtin_amdgcn_rcpf(a2);
+    const float a3 = a2 + we[2], b3 = b2 + we[2] * e2;
+    const float c3 = we[1] * e1 * e1 + we[2] * e2 * e2 - s2;
+    const float disc3 = b3 * b3 - a3 * c3;
+    const float root3 = te[0] + (b3 + __builtin_amdgcn_sqrtf(disc3)) * __builtin_amdgcn_rcpf(a3);
+    const bool ok3 = tn[2] < inf && disc3 >= 0.0f && root3 >= tn[0] && root3 >= tn[1] && root3 >= tn[2];
+    const bool ok2 = tn[1] < inf && disc2 >= 0.0f && root2 >= tn[0] && root2 >= tn[1];
+    const bool ok1 = root1 >= tn[0];
+    const float cand = ok3 ? root3 : ok2 ? root2 : ok1 ? root1 : inf;
+    return cand < inf ? r + cand : inf;
+}
+
+// 1 / x and 1 / sqrt(x) for x > 0: the hardware's estimates and two Newton steps
+__device__ inline double rcp_nr(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return fma(y, fma(-x, y, 1.0), y);
+}
+__device__ inline double rsq_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    return y * fma(-0.5 * x * y, y, 1.5);
+}
+// sqrt(x) for x >= 0 (0 for x <= 0)
+__device__ inline double sqrt_nr(double x) { return x > 0 ? x * rsq_nr(x) : 0.0; }
+
+__device__ inline double godunov_lean64(double s, const double first[3], const double t2[3], const double wa[3][2])
+{
+    const double r = fmin(first[0], fmin(first[1], first[2]));
+    const double inf = INFINITY;
+    if (!(r < inf)) return inf;
+    double te[3], we[3], tn[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const bool on = first[a] < inf;
+        const double of = on ? first[a] - r : inf;
+        const bool so = on && t2[a] <= first[a];
+        te[a] = so ? of + (first[a] - t2[a]) * (1.0 / 3.0) : of;
+        we[a] = so ? wa[a][1] : wa[a][0];
+        tn[a] = of;
+    }
+    if (te[0] > te[1]) { TS_SWAP(te[0], te[1]) TS_SWAP(we[0], we[1]) TS_SWAP(tn[0], tn[1]) }
+    if (te[1] > te[2]) {
+        TS_SWAP(te[1], te[2]) TS_SWAP(we[1], we[2]) TS_SWAP(tn[1], tn[2])
+        if (te[0] > te[1]) { TS_SWAP(te[0], te[1]) TS_SWAP(we[0], we[1]) TS_SWAP(tn[0], tn[1]) }
+    }
+#undef TS_SWAP
+    const double s2 = s * s;
+    const double e1 = te[1] - te[0], e2 = te[2] - te[0];
+    const double root1 = te[0] + s * rsq_nr(we[0]);
+    const double a2 = we[0] + we[1], b2 = we[1] * e1;
+    const double disc2 = a2 * s2 - we[0] * we[1] * e1 * e1;
+    const double root2 = te[0] + (b2 + sqrt_nr(disc2)) * rcp_nr(a2);
+    const double a3 = a2 + we[2], b3 = b2 + we[2] * e2;
+    const double c3 = we[1] * e1 * e1 + we[2] * e2 * e2 - s2;
+    const double disc3 = b3 * b3 - a3 * c3;
+    const double root3 = te[0] + (b3 + sqrt_nr(disc3)) * rcp_nr(a3);
+    const bool ok3 = tn[2] < inf && disc3 >= 0.0 && root3 >= tn[0] && root3 >= tn[1] && root3 >= tn[2];
+    const bool ok2 = tn[1] < inf && disc2 >= 0.0 && root2 >= tn[0] && root2 >= tn[1];
+    const bool ok1 = root1 >= tn[0];
+    const double cand = ok3 ? root3 : ok2 ? root2 : ok1 ? root1 : inf;
+    return cand < inf ? r + cand : inf;
+}
+
 }  // namespace ts
