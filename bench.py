@@ -90,11 +90,11 @@ def cpu_baseline(wl, seeds, mof, budget_s: float = 20.0):
                       f"one field per core, {wall:.1f} s"}
 
 
-def lsqr_bytes_per_iteration(m: int, n: int, nnz: int, k: int, s: int = 4) -> float:
+def lsqr_bytes_per_iteration(m: int, n: int, nnz: int, k: int, s: int = 4, v: int = 4) -> float:
     """Algorithmic HBM bytes of one LSQR iteration over A and its stored transpose for k right-hand sides:
-    both matrices streamed once (FP32 value + int32 index per non-zero, int32 row starts), and per RHS
-    u read + written, the gathered v and u, v, w, x read + written: 2(8 nnz + 4(m+1)) + k s (3m + 7n)."""
-    return 2 * (8 * nnz + 4 * (m + 1)) + k * s * (3 * m + 7 * n)
+    both matrices streamed once (a v-byte value + int32 index per non-zero, int32 row starts), and per RHS
+    u read + written, the gathered v and u, v, w, x read + written: 2((v + 4) nnz + 4(m+1)) + k s (3m + 7n)."""
+    return 2 * ((v + 4) * nnz + 4 * (m + 1)) + k * s * (3 * m + 7 * n)
 
 
 def lsqr_leg(dev, wl, ts, args):
@@ -135,7 +135,7 @@ def lsqr_leg(dev, wl, ts, args):
     damps = 10.0 ** (-1 + 3 * np.arange(k) / max(1, k - 1))
     its = args.lsqr_iterations
     runs = {}
-    for mode in ("fast_mixed", "fast_fp32"):
+    for mode in ("fast_mixed", "fast_fp32", "fast_fp64"):
         t0 = time.perf_counter()
         rc, why, res = Q.solve_many(A, B, damps, its, 0.0, 0.0, 0.0, mode=mode)
         wall = time.perf_counter() - t0
@@ -162,13 +162,18 @@ def lsqr_leg(dev, wl, ts, args):
                                0.0, 0.0, 0.0, mode="plain_fp64")
     x0 = plain_x[picks[0]]
     spread = float(np.linalg.norm(pp[0].solution - x0) / np.linalg.norm(x0))
+    # the FP64 solver's own rounding spread per sampled RHS: the distance between two FP64 solvers that
+    # differ only in summation order (the plain one-RHS form and the FP64-valued SpMM form)
+    fp64_pair = {str(j): float(np.linalg.norm(runs["fast_fp64"][0][j].solution - plain_x[j]) /
+                               max(1e-300, np.linalg.norm(plain_x[j]))) for j in picks}
     plain_rate = plain_its / plain_s                       # RHS-iterations per second
     forms = {}
     for mode, (res, wall, dev_s) in runs.items():
         errs = {str(j): float(np.linalg.norm(res[j].solution - plain_x[j]) / max(1e-300, np.linalg.norm(plain_x[j])))
                 for j in picks}
         it_max = max(r.iterations for r in res)
-        bpi = lsqr_bytes_per_iteration(m, n, nnz, k, 8 if mode == "fast_mixed" else 4)
+        bpi = lsqr_bytes_per_iteration(m, n, nnz, k, 4 if mode == "fast_fp32" else 8,
+                                       8 if mode == "fast_fp64" else 4)
         rate = sum(r.iterations for r in res) / wall
         forms[mode] = {
             "rhs_iterations_per_s": round(rate, 2), "seconds": round(wall, 3), "device_seconds": round(dev_s, 3),
@@ -179,7 +184,7 @@ def lsqr_leg(dev, wl, ts, args):
                          "achieved": round(bpi * it_max / dev_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(bpi * it_max / dev_s / 1e9 / HBM_PEAK_GBS, 4)},
         }
-    main = forms["fast_mixed"]
+    main = forms["fast_fp64"]
     out = {
         "workload": f"G of {used.size} curved rays ({args.lsqr_rays} arrivals, 20,000 events x {ns} stations x P,S) "
                     f"+ first differences on {wl.grid.nx}x{wl.grid.ny}x{wl.grid.nz} (P, S), {k} RHS, "
@@ -187,11 +192,16 @@ def lsqr_leg(dev, wl, ts, args):
         "rows": m, "columns": n, "nnz": nnz,
         "rays_seconds": round(t_rays, 3), "system_seconds": round(t_sys, 3),
         "rhs_iterations_per_s": main["rhs_iterations_per_s"], "unit": "RHS-iterations/s",
-        "form": "fast_mixed (FP32 matrix values, FP64 vectors and scalars)",
+        "form": "fast_fp64 (FP64 matrix values, vectors and scalars: the FP64 solver's arithmetic in the SpMM "
+                "summation order); fast_mixed (FP32 values) and fast_fp32 beside it",
         "speedup_vs_plain_fp64_gpu": main["speedup_vs_plain_fp64_gpu"],
         "model_rel_l2_vs_fp64": main["model_rel_l2_vs_fp64"]["max"],
         "fp64_rounding_spread": {"value": spread, "rhs": picks[0],
                                  "what": "plain FP64 iterate moved by a 1e-15 relative perturbation of b"},
+        "fp64_solver_pair_spread": {"per_rhs": fp64_pair, "max": max(fp64_pair.values()),
+                                    "what": "distance between two FP64 solvers that differ only in summation "
+                                            "order: the plain one-RHS form and the FP64-valued SpMM form"},
+        "damping_per_rhs": {str(j): float(damps[j]) for j in picks},
         "plain_fp64_gpu": {"rhs_iterations_per_s": round(plain_rate, 2), "sample": f"RHS {picks}"},
         "forms": forms,
         "roofline": main["roofline"],
@@ -199,7 +209,8 @@ def lsqr_leg(dev, wl, ts, args):
     errs = [main["model_rel_l2_vs_fp64"]["max"]]
     out["rhs_iterations_per_s"] = main["rhs_iterations_per_s"]
     log(f"lsqr leg: {out['rhs_iterations_per_s']} RHS-it/s, {out['speedup_vs_plain_fp64_gpu']}x plain, "
-        f"max model error {max(errs):.2e} (FP64 spread {spread:.2e}), {out['roofline']['achieved']} GB/s")
+        f"max model error {max(errs):.2e} (FP64 spread {spread:.2e}, FP64 solver pair {max(fp64_pair.values()):.2e}), "
+        f"{out['roofline']['achieved']} GB/s")
     A.release()
     return out
 

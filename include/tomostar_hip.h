@@ -55,6 +55,10 @@ extern "C" {
 #define TS_LSQR_FAST_MIXED 2 /* as TS_LSQR_FAST_FP32 with FP64 u/v/w: only the matrix values are FP32.
                                 FP64 operands keep the iterates on the FP64 solver's path; FP32 operands
                                 of an ill-conditioned system drift by ~1e-3 within tens of iterations */
+#define TS_LSQR_FAST_FP64 3  /* the multi-RHS SpMM form with FP64 matrix values and FP64 u/v/w: the FP64
+                                solver's arithmetic in another summation order (the matrix must be created
+                                with keep_fp64); for weakly damped systems, where FP32 values move the
+                                model by more than the FP64 solver's own rounding spread */
 
 typedef struct ts_context ts_context;
 typedef struct ts_tableset ts_tableset;

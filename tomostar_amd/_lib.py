@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("TOMOSTAR_LIB") or os.path.join(HERE, "libtomostar_hip
 TS_OK, TS_DECLINED, TS_EINVAL, TS_CANCELLED, TS_EDEVICE = 0, 1, -1, -2, -3
 TS_CREATE_SKIP_SELFTEST = 1
 TS_EIK_PLAIN_FP64, TS_EIK_FAST_FP32, TS_EIK_FAST_FP64 = 0, 1, 2
-TS_LSQR_PLAIN_FP64, TS_LSQR_FAST_FP32, TS_LSQR_FAST_MIXED = 0, 1, 2
+TS_LSQR_PLAIN_FP64, TS_LSQR_FAST_FP32, TS_LSQR_FAST_MIXED, TS_LSQR_FAST_FP64 = 0, 1, 2, 3
 
 
 class TsGrid(C.Structure):

@@ -283,9 +283,9 @@ void matrix_finish(ts_matrix &A, dbuf<double> &&d_va64, bool keep_fp64, const in
 
 // Y[r][j] = sc[j] * sum_q va[q] X[ci[q]][j] - sc[KP + j] * Y[r][j] for all rows, FP64 row sums; the
 // squares of the stored results are summed per lane and reduced per block in a fixed order.
-template <int KP, typename T>
+template <int KP, typename T, typename V>
 __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__restrict__ rs,
-                                                 const int32_t *__restrict__ ci, const float *__restrict__ va,
+                                                 const int32_t *__restrict__ ci, const V *__restrict__ va,
                                                  const T *__restrict__ X, T *__restrict__ Y,
                                                  const double *__restrict__ sc, double *__restrict__ partial)
 {
@@ -310,18 +310,18 @@ __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__rest
             constexpr int kBatch = sizeof(T) == 8 ? kGatherBatch : kGatherBatch / 2;
             constexpr int GB = KP < kBatch ? KP : kBatch;
             int cn = 0;
-            float vn = 0.0f;
+            V vn = 0;
             if (j < b - a) { cn = ci[a + j]; vn = va[a + j]; }
             for (int q0 = a; q0 < b; q0 += KP) {
                 const int cnt = min(KP, b - q0);
                 const int c = cn;
-                const float v = vn;
+                const V v = vn;
                 // the next chunk's indices and values load while this chunk gathers
                 if (q0 + KP + j < b) { cn = ci[q0 + KP + j]; vn = va[q0 + KP + j]; }
                 int t = 0;
                 for (; t + GB <= cnt; t += GB) {
                     T x[GB];
-                    float w[GB];
+                    V w[GB];
 #pragma unroll
                     for (int u = 0; u < GB; u++) {
                         const int col = __shfl(c, t + u, KP);
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__rest
             int q = a;
             for (; q + 4 <= b; q += 4) {
                 const int c0 = ci[q], c1 = ci[q + 1], c2 = ci[q + 2], c3 = ci[q + 3];
-                const float v0 = va[q], v1 = va[q + 1], v2 = va[q + 2], v3 = va[q + 3];
+                const V v0 = va[q], v1 = va[q + 1], v2 = va[q + 2], v3 = va[q + 3];
                 const T x0 = X[(size_t)c0 * KP + j], x1 = X[(size_t)c1 * KP + j];
                 const T x2 = X[(size_t)c2 * KP + j], x3 = X[(size_t)c3 * KP + j];
                 acc0 += (double)v0 * (double)x0;
@@ -373,9 +373,9 @@ __global__ __launch_bounds__(256) void lsqr_spmm(int rows, const int32_t *__rest
 // (8 / KP)-th entry of the row; the lanes' sums meet in a fixed shuffle tree. A row of a few hundred
 // entries (a ray among regularisation rows) then takes a few dozen dependent steps instead of holding
 // its wave for the whole row. Same outputs and partials as lsqr_spmm, in another summation order.
-template <int KP, typename T>
+template <int KP, typename T, typename V>
 __global__ __launch_bounds__(256) void lsqr_spmm_split(int rows, const int32_t *__restrict__ rs,
-                                                       const int32_t *__restrict__ ci, const float *__restrict__ va,
+                                                       const int32_t *__restrict__ ci, const V *__restrict__ va,
                                                        const T *__restrict__ X, T *__restrict__ Y,
                                                        const double *__restrict__ sc, double *__restrict__ partial)
 {
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(256) void lsqr_spmm_split(int rows, const int32_t *
         int q = a + sub;
         for (; q + 3 * L < b; q += 4 * L) {
             const int c0 = ci[q], c1 = ci[q + L], c2 = ci[q + 2 * L], c3 = ci[q + 3 * L];
-            const float v0 = va[q], v1 = va[q + L], v2 = va[q + 2 * L], v3 = va[q + 3 * L];
+            const V v0 = va[q], v1 = va[q + L], v2 = va[q + 2 * L], v3 = va[q + 3 * L];
             const T x0 = X[(size_t)c0 * KP + j], x1 = X[(size_t)c1 * KP + j];
             const T x2 = X[(size_t)c2 * KP + j], x3 = X[(size_t)c3 * KP + j];
             acc0 += (double)v0 * (double)x0;
@@ -429,12 +429,20 @@ __global__ __launch_bounds__(256) void lsqr_spmm_split(int rows, const int32_t *
 }
 
 // One product: lsqr_spmm, or lsqr_spmm_split below 8 right-hand sides.
-template <int KP, typename T>
-void spmm(hipStream_t st, int rows, const int32_t *rs, const int32_t *ci, const float *va, const T *X, T *Y,
+template <int KP, typename T, typename V>
+void spmm(hipStream_t st, int rows, const int32_t *rs, const int32_t *ci, const V *va, const T *X, T *Y,
           const double *sc, double *partial)
 {
-    if constexpr (KP < 8) lsqr_spmm_split<KP, T><<<kSpmmBlocks, 256, 0, st>>>(rows, rs, ci, va, X, Y, sc, partial);
-    else lsqr_spmm<KP, T><<<kSpmmBlocks, 256, 0, st>>>(rows, rs, ci, va, X, Y, sc, partial);
+    if constexpr (KP < 8) lsqr_spmm_split<KP, T, V><<<kSpmmBlocks, 256, 0, st>>>(rows, rs, ci, va, X, Y, sc, partial);
+    else lsqr_spmm<KP, T, V><<<kSpmmBlocks, 256, 0, st>>>(rows, rs, ci, va, X, Y, sc, partial);
+}
+
+// The matrix's values in the type V: FP32 (always kept) or FP64 (kept when the matrix was created so).
+template <typename V>
+const V *values_of(const ts_matrix &A, bool transpose)
+{
+    if constexpr (sizeof(V) == 8) return transpose ? A.tva64.p : A.va64.p;
+    else return transpose ? A.tva32.p : A.va32.p;
 }
 
 // x += cx w ; w = cv V + cw w over n rows; squares of the new w per block (for the next ddnorm).
@@ -849,18 +857,18 @@ struct FastLsqr {
     dbuf<LsqrScal> S;
 };
 
-template <int KP, typename T>
+template <int KP, typename T, typename V>
 void fast_iteration(ts_matrix &A, FastLsqr<T> &F, hipStream_t st, bool with_hist)
 {
-    spmm<KP, T>(st, A.rows, A.rs.p, A.ci.p, A.va32.p, F.V.p, F.U.p, F.sc_a.p, F.pu.p);
+    spmm<KP, T, V>(st, A.rows, A.rs.p, A.ci.p, values_of<V>(A, false), F.V.p, F.U.p, F.sc_a.p, F.pu.p);
     lsqr_fin_u<<<1, kFinThreads, 0, st>>>(F.k, KP, F.S.p, F.pu.p, kSpmmBlocks, F.sc_t.p);
-    spmm<KP, T>(st, A.cols, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
+    spmm<KP, T, V>(st, A.cols, A.trs.p, A.tci.p, values_of<V>(A, true), F.U.p, F.V.p, F.sc_t.p, F.pv.p);
     lsqr_fin_v<<<1, kFinThreads, 0, st>>>(F.k, KP, F.S.p, F.pv.p, F.pw.p, kSpmmBlocks, kSpmmBlocks, F.sc_a.p,
                                  with_hist ? F.hist.p : nullptr, F.max_it);
     lsqr_xw<KP, T><<<kSpmmBlocks, 256, 0, st>>>(A.cols, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
 }
 
-template <int KP, typename T>
+template <int KP, typename T, typename V>
 void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int max_it, double atol, double btol,
                   double conlim, const volatile int32_t *cancel, SolveOut o, double *timing)
 {
@@ -918,7 +926,7 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     for (int j = 0; j < KP; j++) { sc[j] = S[j].done ? 0.0 : S[j].su; sc[KP + j] = 0; }
     TS_HIP(hipMemcpyAsync(F.sc_t.p, sc.data(), 2 * KP * 8, hipMemcpyHostToDevice, st));
     // v = A^T u, alpha, w = v
-    spmm<KP, T>(st, n, A.trs.p, A.tci.p, A.tva32.p, F.U.p, F.V.p, F.sc_t.p, F.pv.p);
+    spmm<KP, T, V>(st, n, A.trs.p, A.tci.p, values_of<V>(A, true), F.U.p, F.V.p, F.sc_t.p, F.pv.p);
     lsqr_fin_init<<<1, kFinThreads, 0, st>>>(k, KP, F.S.p, F.pv.p, kSpmmBlocks, F.sc_a.p);
     lsqr_xw<KP, T><<<kSpmmBlocks, 256, 0, st>>>(n, F.V.p, F.W.p, F.X.p, F.S.p, F.pw.p);
     TS_HIP(hipGetLastError());
@@ -930,7 +938,7 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     if (A.ctx->use_graphs && !stream_override()) {
         hipGraph_t gr;
         TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        fast_iteration<KP, T>(A, F, st, o.hist != nullptr);
+        fast_iteration<KP, T, V>(A, F, st, o.hist != nullptr);
         TS_HIP(hipStreamEndCapture(st, &gr));
         hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
@@ -950,7 +958,7 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
         int todo = std::min(check, max_it - it);
         for (int q = 0; q < todo; q++) {
             if (ge) TS_HIP(hipGraphLaunch(ge, st));
-            else fast_iteration<KP, T>(A, F, st, o.hist != nullptr);
+            else fast_iteration<KP, T, V>(A, F, st, o.hist != nullptr);
         }
         it += todo;
         TS_HIP(hipMemcpyAsync(cur.data(), F.S.p, KP * sizeof(LsqrScal), hipMemcpyDeviceToHost, st));
@@ -978,25 +986,29 @@ void lsqr_fast_kp(ts_matrix &A, int k, const double *b, const double *damp, int 
     }
 }
 
-template <typename T>
+template <typename T, typename V>
 void lsqr_fast_t(ts_matrix &A, int k, const double *b, const double *damp, int max_it, double atol, double btol,
                  double conlim, const volatile int32_t *cancel, SolveOut o, double *timing)
 {
-    if (k <= 1) lsqr_fast_kp<1, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 2) lsqr_fast_kp<2, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 4) lsqr_fast_kp<4, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 8) lsqr_fast_kp<8, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 16) lsqr_fast_kp<16, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else if (k <= 32) lsqr_fast_kp<32, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else lsqr_fast_kp<64, T>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    if (k <= 1) lsqr_fast_kp<1, T, V>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 2) lsqr_fast_kp<2, T, V>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 4) lsqr_fast_kp<4, T, V>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 8) lsqr_fast_kp<8, T, V>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 16) lsqr_fast_kp<16, T, V>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else if (k <= 32) lsqr_fast_kp<32, T, V>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else lsqr_fast_kp<64, T, V>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
 }
 
-// vectors in FP32 (TS_LSQR_FAST_FP32) or FP64 (TS_LSQR_FAST_MIXED); matrix values FP32 in both
+// vectors in FP32 (TS_LSQR_FAST_FP32) or FP64 (TS_LSQR_FAST_MIXED, TS_LSQR_FAST_FP64); matrix values FP32,
+// or FP64 in TS_LSQR_FAST_FP64 (the matrix must keep them)
 void lsqr_fast(ts_matrix &A, int k, const double *b, const double *damp, int max_it, double atol, double btol,
-               double conlim, const volatile int32_t *cancel, SolveOut o, double *timing, bool fp64_vectors)
+               double conlim, const volatile int32_t *cancel, SolveOut o, double *timing, int mode)
 {
-    if (fp64_vectors) lsqr_fast_t<double>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
-    else lsqr_fast_t<float>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    if (mode == TS_LSQR_FAST_FP64) {
+        if (!A.va64.p || !A.tva64.p) fail(TS_EINVAL, "the FP64-valued form needs a matrix created with its FP64 values kept");
+        lsqr_fast_t<double, double>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    } else if (mode == TS_LSQR_FAST_MIXED) lsqr_fast_t<double, float>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
+    else lsqr_fast_t<float, float>(A, k, b, damp, max_it, atol, btol, conlim, cancel, o, timing);
 }
 
 // ------------------------------------------------------------------------------- self-test
@@ -1138,7 +1150,8 @@ int lsqr_solve_core(ts_matrix *a, int k, const double *b, const double *damp, in
                     char *reason, int reason_len)
 {
     if (!a || k < 1 || k > 64 || !b || !damp || max_iterations < 0 || !x || !iterations || !stop || !stats ||
-        (mode != TS_LSQR_PLAIN_FP64 && mode != TS_LSQR_FAST_FP32 && mode != TS_LSQR_FAST_MIXED)) {
+        (mode != TS_LSQR_PLAIN_FP64 && mode != TS_LSQR_FAST_FP32 && mode != TS_LSQR_FAST_MIXED &&
+         mode != TS_LSQR_FAST_FP64)) {
         set_reason(reason, reason_len, "invalid argument");
         return TS_EINVAL;
     }
@@ -1155,8 +1168,7 @@ int lsqr_solve_core(ts_matrix *a, int k, const double *b, const double *damp, in
         }
         SolveOut o{x, iterations, stop, stats, rnorm_history};
         if (mode == TS_LSQR_PLAIN_FP64) lsqr_plain(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o);
-        else lsqr_fast(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o, seconds,
-                       mode == TS_LSQR_FAST_MIXED);
+        else lsqr_fast(*a, k, b, damp, max_iterations, atol, btol, conlim, cancel, o, seconds, mode);
         return TS_OK;
     });
 }

@@ -306,7 +306,8 @@ int ts_server_submit_lsqr(ts_server *S, ts_matrix *a, int k, const double *b, co
                           int reason_len)
 {
     if (!S || !a || k < 1 || k > 64 || !b || !damp || max_iterations < 0 || !request_id ||
-        (mode != TS_LSQR_PLAIN_FP64 && mode != TS_LSQR_FAST_FP32 && mode != TS_LSQR_FAST_MIXED)) {
+        (mode != TS_LSQR_PLAIN_FP64 && mode != TS_LSQR_FAST_FP32 && mode != TS_LSQR_FAST_MIXED &&
+         mode != TS_LSQR_FAST_FP64)) {
         set_reason(reason, reason_len, "invalid argument");
         return TS_EINVAL;
     }

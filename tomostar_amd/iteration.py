@@ -70,7 +70,7 @@ def observed_times(dev: E.Device, wl: Workload, eik_mode: str = "fast_fp64") -> 
 
 
 def run(dev: E.Device, wl: Workload, t_obs: np.ndarray, settings: Sy.SystemSettings | None = None,
-        damping: float = 1.0, smoothing: float = 5.0, lsqr_iterations: int = 100, lsqr_mode: str = "fast_mixed",
+        damping: float = 1.0, smoothing: float = 5.0, lsqr_iterations: int = 100, lsqr_mode: str = "fast_fp64",
         eik_mode: str = "fast_fp32", sigma: float = 0.05, keep_fp64: bool = True) -> IterationResult:
     """One iteration from the 1-D starting model; LSQR with damp 0 and a fixed iteration budget
     (atol = btol = conlim = 0). Stage wall times in stage_seconds."""

@@ -14,10 +14,11 @@ from typing import Callable, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import TS_LSQR_FAST_FP32, TS_LSQR_FAST_MIXED, TS_LSQR_PLAIN_FP64, check, ptr, reason_buffer
+from ._lib import TS_LSQR_FAST_FP32, TS_LSQR_FAST_FP64, TS_LSQR_FAST_MIXED, TS_LSQR_PLAIN_FP64, check, ptr, reason_buffer
 from .eikonal import Device
 
-MODES = {"plain_fp64": TS_LSQR_PLAIN_FP64, "fast_fp32": TS_LSQR_FAST_FP32, "fast_mixed": TS_LSQR_FAST_MIXED}
+MODES = {"plain_fp64": TS_LSQR_PLAIN_FP64, "fast_fp32": TS_LSQR_FAST_FP32, "fast_mixed": TS_LSQR_FAST_MIXED,
+         "fast_fp64": TS_LSQR_FAST_FP64}
 
 # Lsqr.cs:141-147 and :48-62, in the ABI's stop-code order
 STOP_REASONS = {
@@ -86,6 +87,7 @@ class Matrix:
             raise _lib.HipError(rc, r.value.decode(errors="replace"))
         self._h = h
         self.nnz = int(self._va.size)
+        self.has_fp64 = bool(keep_fp64)
 
     @staticmethod
     def from_scipy(device: Device, a, keep_fp64: bool = True) -> "Matrix":
@@ -135,11 +137,19 @@ def multiply(a: Matrix, x: np.ndarray, transpose: bool = False) -> np.ndarray:
     return y[0] if single else y
 
 
+def default_mode(a: Matrix) -> str:
+    """The fast form that keeps the FP64 solver's arithmetic: FP64 matrix values when the matrix keeps them
+    (TS_LSQR_FAST_FP64), else FP32 values with FP64 vectors (TS_LSQR_FAST_MIXED)."""
+    return "fast_fp64" if getattr(a, "has_fp64", False) else "fast_mixed"
+
+
 def solve_many(a: Matrix, b: np.ndarray, damp: Sequence[float], max_iterations: int = 500, atol: float = 1e-8,
-               btol: float = 1e-8, conlim: float = 1e8, mode: str = "fast_fp32", cancel: C.c_int32 | None = None,
+               btol: float = 1e-8, conlim: float = 1e8, mode: str | None = None, cancel: C.c_int32 | None = None,
                history: bool = False):
-    """k right-hand sides b[k, m] with damping damp[k]; returns (status, reason, [LsqrResult] * k)."""
+    """k right-hand sides b[k, m] with damping damp[k]; returns (status, reason, [LsqrResult] * k).
+    mode None: default_mode(a)."""
     L = _declare()
+    mode = default_mode(a) if mode is None else mode
     b = np.ascontiguousarray(np.atleast_2d(b), np.float64)
     k = b.shape[0]
     if b.shape[1] != a.rows:
@@ -169,13 +179,15 @@ class LsqrHip:
     SelfTestTolerance = 1e-8  # LsqrOpenCl.cs:33
     MinNonZerosForDevice = 2_000_000  # LsqrOpenCl.cs:36
 
-    def __init__(self, device: Device, mode: str = "fast_fp32"):
+    def __init__(self, device: Device, mode: str | None = None):
+        """mode None: per matrix, the form that keeps the FP64 solver's arithmetic (default_mode), as
+        LsqrOpenCl computes in FP64."""
         self._dev = device
         self.mode = mode
 
     @staticmethod
     def try_create(log: Callable[[str], None] | None = None, device: int = 0,
-                   mode: str = "fast_fp32") -> "LsqrHip | None":
+                   mode: str | None = None) -> "LsqrHip | None":
         try:
             dev = Device.open(device, self_test=True)
         except _lib.HipError as e:

@@ -200,6 +200,7 @@ def build_qsystem(device: Device, grid: GridDefinition, settings: QTomographySet
     if rc != 0:
         raise _lib.HipError(rc, r.value.decode(errors="replace"))
     mat = _AdoptedMatrix(device, h)
+    mat.has_fp64 = bool(keep_fp64)
     rhs = np.zeros(max(1, mat.rows))
     check(_lib.load().ts_matrix_read(h, None, None, None, ptr(rhs)))
     names = ["off_station", "columns", "rows", "data_rows"]

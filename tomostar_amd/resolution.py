@@ -34,7 +34,7 @@ def probe_signs(n: int, probes: int, seed: int) -> np.ndarray:
 
 
 def estimate(a: Q.Matrix, data_rows: int, probes: int, lsqr_iterations: int, tolerance: float, seed: int,
-             mode: str = "fast_mixed", batch: int = 64, signs: np.ndarray | None = None) -> ResolutionDiagonalResult:
+             mode: str | None = None, batch: int = 64, signs: np.ndarray | None = None) -> ResolutionDiagonalResult:
     """ResolutionDiagonal.Estimate on the device: probes >= 2 (as the reference clamps), LSQR with damp 0,
     atol = btol = tolerance, conlim 1e8 per probe, up to `batch` probes per multi-RHS solve. `signs`
     (probes x cols of +-1) replaces the generated probes."""

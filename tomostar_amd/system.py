@@ -112,6 +112,7 @@ class _AdoptedMatrix(Matrix):
         check(L.ts_matrix_shape(handle, C.byref(rows), C.byref(cols), C.byref(nnz), C.byref(secs)))
         self.rows, self.cols, self.nnz = rows.value, cols.value, nnz.value
         self.build_seconds = secs.value
+        self.has_fp64 = False  # set by the builder that knows
 
 
 def build_system(device: Device, grid: GridDefinition, settings: SystemSettings, rays: RayBatch, ray_of_row,
@@ -155,4 +156,5 @@ def build_system(device: Device, grid: GridDefinition, settings: SystemSettings,
     rhs = np.zeros(max(1, mat.rows))
     check(L.ts_matrix_read(h, None, None, None, ptr(rhs)))
     names = ["offP", "offS", "offH", "offCP", "offCS", "columns", "rows", "data_rows"]
+    mat.has_fp64 = bool(keep_fp64)
     return System(mat, rhs[:mat.rows], dict(zip(names, (int(x) for x in layout))), mat.build_seconds)

@@ -97,7 +97,7 @@ def trade_off_point(g: GridDefinition, settings: Sy.SystemSettings, sysd: Sy.Sys
 
 def trade_off(dev: E.Device, wl: Workload, t_obs: np.ndarray, pairs, settings: Sy.SystemSettings | None = None,
               lsqr_iterations: int = 100, lsqr_tolerance: float = 1e-6, sigma: float = 0.05,
-              lsqr_mode: str = "fast_mixed", eik_mode: str = "fast_fp32") -> list[TradeOffPoint]:
+              lsqr_mode: str = "fast_fp64", eik_mode: str = "fast_fp32") -> list[TradeOffPoint]:
     """TradeOff for the (damping, smoothing) pairs from the workload's 1-D starting model: LSQR with
     damp 0, LsqrIterations, atol = btol = LsqrTolerance, conlim 1e8 per pair (:597)."""
     settings = settings or Sy.SystemSettings(vpvs=True)
