@@ -65,8 +65,23 @@ def host_cpu() -> dict:
             "omp_num_threads": share or None}
 
 
-def cpu_baseline(wl, seeds, mof, budget_s: float = 20.0):
-    """The reference's CPU path restated (fast marching, one table per core) on a bounded sample."""
+def _field_errors(dev, ref, seed_idx):
+    """max and RMS relative error over the computed nodes outside the seeds (t > 0), and the device
+    gate's range-normalised metric max |dt| / (max t - min t) (EikonalOpenCl.cs:371)."""
+    mask = np.isfinite(ref) & (ref > 0)
+    mask[seed_idx] = False
+    r = np.abs(dev[mask] - ref[mask]) / ref[mask]
+    fin = np.isfinite(ref)
+    span = float(ref[fin].max() - ref[fin].min())
+    return {"max_rel": float(r.max()), "rms_rel": float(np.sqrt(np.mean(r * r))),
+            "range_normalised": float(np.max(np.abs(dev[fin] - ref[fin])) / span)}
+
+
+def cpu_baseline(wl, seeds, mof, budget_s: float = 20.0, compare=None):
+    """The reference's CPU path restated (fast marching, one table per core) on a bounded sample. With
+    compare = {field: (mixed table, FP64-form table)}, the fast-marching tables of those fields (within
+    the sample) are also the reference the device's tables are checked against: FMM is what the
+    reference's CPU workers produce (TravelTimeTable.cs:341-354), while the device forms follow Relax."""
     from oracle import oracle as O
 
     g = wl.grid
@@ -81,13 +96,22 @@ def cpu_baseline(wl, seeds, mof, budget_s: float = 20.0):
     nf = int(max(cores, min(len(seeds), cores * max(1, round(budget_s / max(one, 1e-3))))))
     nf = min(nf, len(seeds))
     t0 = time.perf_counter()
-    O.fmm_batch(og, [models[m] for m in mof[:nf]], seeds[:nf], threads=cores)
+    tabs = O.fmm_batch(og, [models[m] for m in mof[:nf]], seeds[:nf], threads=cores)
     wall = time.perf_counter() - t0
+    against_fmm = None
+    if compare:
+        tabs = np.asarray(tabs).reshape(nf, -1)
+        against_fmm = {}
+        for f, (mixed, fp64) in compare.items():
+            if f < nf:
+                against_fmm[str(f)] = {"mixed_vs_fmm": _field_errors(mixed, tabs[f], seeds[f][0]),
+                                       "relax_vs_fmm": _field_errors(fp64, tabs[f], seeds[f][0])}
     return {"value": nf * g.count / wall / 1e9, "unit": "Gnode/s", "cores": cores, "kind": "port",
             "nproc": host["nproc"], "affinity_cpus": host["affinity"], "omp_num_threads": host["omp_num_threads"],
             "cpu_model": host["model"],
             "sample": f"{nf} of {len(seeds)} fields, fast marching (SphericalEikonal.Solve restated in C), "
-                      f"one field per core, {wall:.1f} s"}
+                      f"one field per core, {wall:.1f} s",
+            "device_tables_vs_fmm": against_fmm}
 
 
 def lsqr_bytes_per_iteration(m: int, n: int, nnz: int, k: int, s: int = 4, v: int = 4) -> float:
@@ -488,6 +512,7 @@ def uneven_leg(dev, args):
         times.append(time.perf_counter() - t0)
     mixed = [b.read(q) for q in range(len(probs))]
     rounds = b.rounds.copy()
+    st = b.stats()
     t0 = time.perf_counter()
     rc, why = b.solve("fast_fp64")
     t64 = time.perf_counter() - t0
@@ -521,6 +546,15 @@ def uneven_leg(dev, args):
            "max_rel_error_vs_fp64": max(errs), "rounds_max": [int(rounds[:, 0].max()), int(rounds[:, 1].max())],
            "plain_fp64_gpu": {"value": round(plain, 4), "unit": "Gnode/s", "sample": f"problems {sample}"},
            "speedup_vs_plain_fp64_gpu": round(value / plain, 2)}
+    # the sweep kernel's roofline: algorithmic bytes of the executed updates (12 B on FP32 times, 20 B on
+    # FP64 times) over the sweep time of the last timed solve (HIP events around the rounds)
+    alg = BYTES_PER_UPDATE_FP32 * st["updates_fp32"] + BYTES_PER_UPDATE_FP64 * st["updates_fp64"]
+    if st["sweep_seconds"] > 0:
+        ach = alg / st["sweep_seconds"] / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "mg_plane",
+                           "units_per_step": {"fp32_updates": st["updates_fp32"], "fp64_updates": st["updates_fp64"]},
+                           "sweep_seconds": round(st["sweep_seconds"], 4)}
     log(f"uneven leg: {out}")
     return out
 
@@ -771,6 +805,27 @@ def main() -> int:
     achieved = dom_bytes / dom_secs / 1e9
     traffic = TRAFFIC_BYTES_PER_LAUNCH.get(dom)
 
+    # accuracy of the timed (mixed) form per field: against the FP64 form, which is Relax's result to the
+    # last bit, on the first 16 fields (8 stations x P, S); the table set is then solved again in the
+    # timed form for the legs below
+    acc_fields = list(range(min(16, nf)))
+    mixed_tab = {f: ts.read(f, 1)[0] for f in acc_fields}
+    rc, why = ts.solve("fast_fp64")
+    if rc != 0:
+        raise RuntimeError(f"FP64 solve declined: {why}")
+    fp64_tab = {f: ts.read(f, 1)[0] for f in acc_fields}
+    per_field = {str(f): _field_errors(mixed_tab[f], fp64_tab[f], seeds[f][0]) for f in acc_fields}
+    accuracy = {"reference": "the FP64 form (SphericalEikonal.Relax's arithmetic and schedule, bit for bit)",
+                "fields": len(acc_fields), "per_field": per_field,
+                "max_rel": max(v["max_rel"] for v in per_field.values()),
+                "rms_rel": float(np.sqrt(np.mean([v["rms_rel"] ** 2 for v in per_field.values()]))),
+                "range_normalised": max(v["range_normalised"] for v in per_field.values())}
+    log(f"accuracy: mixed vs FP64 form over {len(acc_fields)} fields: max {accuracy['max_rel']:.2e}, "
+        f"rms {accuracy['rms_rel']:.2e}, range-normalised {accuracy['range_normalised']:.2e}")
+    rc, why = ts.solve("fast_fp32")
+    if rc != 0:
+        raise RuntimeError(f"solve declined: {why}")
+
     # plain FP64 one-plane-per-launch form on a bounded sample (one of its batches)
     pf = min(args.plain_fields, nf)
     plain_rate, plain_s = None, None
@@ -787,7 +842,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(wl, seeds, mof)
+        cpu = cpu_baseline(wl, seeds, mof, compare={f: (mixed_tab[f], fp64_tab[f]) for f in acc_fields})
         log(f"cpu baseline: {cpu}")
 
     lsqr = lsqr_leg(dev, wl, ts, args) if args.lsqr_rays > 0 else None
@@ -820,6 +875,7 @@ def main() -> int:
         "plain_fp64_gpu": {"value": round(plain_rate, 4), "unit": "Gnode/s", "sample": f"{pf} fields, one batch"}
                           if plain_rate else None,
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "accuracy": accuracy,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": dom,

@@ -163,6 +163,9 @@ int ts_gridbatch_solve(ts_gridbatch *b, int mode, const volatile int32_t *cancel
 int ts_gridbatch_read(ts_gridbatch *b, int problem, double *out);
 /* Total nodes, the last solve's wall time and the device time of its sweep rounds (HIP events). */
 int ts_gridbatch_stats(ts_gridbatch *b, int64_t *nodes, double *seconds, double *sweep_seconds);
+/* Node updates the last solve executed (a dirty, unseeded node evaluated), on FP32 and on FP64 times:
+ * the work behind the sweep time (EikonalOpenCl.LastStats, EikonalOpenCl.cs:77). */
+int ts_gridbatch_updates(ts_gridbatch *b, double *fp32, double *fp64);
 void ts_gridbatch_release(ts_gridbatch *b);
 
 /* ---------------------------------------------------------------- rays

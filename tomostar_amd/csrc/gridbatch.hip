@@ -68,6 +68,8 @@ struct ts_gridbatch {
     dbuf<uint8_t> dirty, seeded;
     dbuf<MgProblem> prob_dev;
     dbuf<int32_t> phase, changed, rounds, status, reset, cap;
+    dbuf<unsigned long long> evals;       // executed updates of the last solve: FP32, FP64
+    double updates[2] = {0, 0};
     dbuf<MgSeg> segs;
     std::vector<int> seg_off;             // per plane index: first segment
     std::vector<int> seg_blocks;          // per plane index: blocks of its launch
@@ -125,10 +127,12 @@ struct MgArgs {
     const int32_t *phase;
     int32_t *changed;
     double wake_tol;  // a drop wakes the node's readers when larger than this (0: any drop)
+    unsigned long long *evals;  // executed updates (one counter per width)
 };
 
+// Returns whether the node was evaluated (dirty, not a seed, positive slowness).
 template <typename R>
-__device__ inline void mg_update(const MgArgs<R> &A, const MgProblem &P, int q, int second, int dir, uint32_t pk)
+__device__ inline bool mg_update(const MgArgs<R> &A, const MgProblem &P, int q, int second, int dir, uint32_t pk)
 {
     const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = nx * ny;
     const int ip = pk & 2047, jp = (pk >> 11) & 2047, kp = pk >> 22;
@@ -137,10 +141,10 @@ __device__ inline void mg_update(const MgArgs<R> &A, const MgProblem &P, int q, 
     const int k = (dir & 4) ? nz - 1 - kp : kp;
     const int node = (k * ny + j) * nx + i;
     const int64_t at = P.node_base + node;
-    if (A.seeded[at] || !A.dirty[at]) return;
+    if (A.seeded[at] || !A.dirty[at]) return false;
     A.dirty[at] = 0;
     const R s = A.slow[at];
-    if (!(s > (R)0)) return;
+    if (!(s > (R)0)) return false;
     R *Tp = A.T + P.node_base;
     const R cur = Tp[node];
     const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
@@ -164,7 +168,7 @@ __device__ inline void mg_update(const MgArgs<R> &A, const MgProblem &P, int q, 
                                  {A.wlat[P.wlat_base + 2 * k], A.wlat[P.wlat_base + 2 * k + 1]}, {P.wr0, P.wr1}};
         next = godunov_abs_w<double>(s, first, t2, wa);
     }
-    if (!(next < cur)) return;
+    if (!(next < cur)) return true;
     Tp[node] = next;
     bool moved;
     if constexpr (sizeof(R) == 4)
@@ -172,7 +176,7 @@ __device__ inline void mg_update(const MgArgs<R> &A, const MgProblem &P, int q, 
     else
         moved = !isfinite(cur) || (double)(cur - next) > kMgSettleTol;
     if (moved) atomicOr(&A.changed[q], 1);
-    if (isfinite(cur) && !((double)(cur - next) > A.wake_tol)) return;
+    if (isfinite(cur) && !((double)(cur - next) > A.wake_tol)) return true;
     uint8_t *d = A.dirty + P.node_base;
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
@@ -181,6 +185,7 @@ __device__ inline void mg_update(const MgArgs<R> &A, const MgProblem &P, int q, 
         if (pos[ax] + 1 < n[ax]) d[node + st[ax]] = 1;
         if (pos[ax] + 2 < n[ax]) d[node + 2 * st[ax]] = 1;
     }
+    return true;
 }
 
 // One sweep step over every problem: block b serves the segment whose block range holds b.
@@ -188,7 +193,9 @@ template <typename R>
 __global__ __launch_bounds__(kMgBlock) void mg_plane(MgArgs<R> A, const MgSeg *__restrict__ segs, int nsegs, int dir)
 {
     __shared__ MgSeg sg;
+    __shared__ unsigned block_evals;
     if (threadIdx.x == 0) {
+        block_evals = 0;
         int lo = 0, hi = nsegs - 1;
         while (lo < hi) {  // the last segment whose first block is <= blockIdx.x
             const int mid = (lo + hi + 1) >> 1;
@@ -205,10 +212,15 @@ __global__ __launch_bounds__(kMgBlock) void mg_plane(MgArgs<R> A, const MgSeg *_
     const int per = P.nodes_per_thread;
     const int t = (blockIdx.x - sg.block_start) * kMgBlock + threadIdx.x;
     const int width = ((sg.count + per - 1) / per + kMgBlock - 1) / kMgBlock * kMgBlock;  // threads of the segment
+    unsigned done = 0;
     for (int r = 0; r < per; r++) {
         const int a = t + r * width;
-        if (a < sg.count) mg_update<R>(A, P, q, ph, dir, A.planes[sg.list_off + a]);
+        if (a < sg.count) done += mg_update<R>(A, P, q, ph, dir, A.planes[sg.list_off + a]);
     }
+    // executed updates: one count per block
+    if (done) atomicAdd(&block_evals, done);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_evals) atomicAdd(&A.evals[sizeof(R) == 8], (unsigned long long)block_evals);
 }
 
 // After a round (one thread per problem): a problem whose round moved no node by more than the
@@ -371,6 +383,7 @@ MgArgs<R> mg_args(ts_gridbatch &b)
     A.phase = b.phase.p;
     A.changed = b.changed.p;
     A.wake_tol = b.wake_tol[sizeof(R) == 8];
+    A.evals = b.evals.p;
     return A;
 }
 
@@ -512,6 +525,7 @@ void mg_solve(ts_gridbatch &b, int mode, const volatile int32_t *cancel)
     int max_rounds = 0;
     for (const Grid &g : b.grids) max_rounds = std::max(max_rounds, 2 * mg_round_cap(g) + 2);
     b.sweep_seconds = 0;
+    TS_HIP(hipMemsetAsync(b.evals.p, 0, 2 * sizeof(unsigned long long), st));
     b.wake_tol[0] = 0.0;
     b.wake_tol[1] = mixed ? kMgMixedWakeTol : 0.0;
     mg_start(b, mixed, st);
@@ -698,6 +712,7 @@ int ts_gridbatch_create(ts_context *ctx, int nproblems, const ts_grid *grids, co
         b.prob_dev.alloc(nproblems);
         b.phase.alloc(nproblems); b.changed.alloc(nproblems); b.rounds.alloc(2 * nproblems);
         b.reset.alloc(nproblems); b.cap.alloc(nproblems); b.status.alloc(2);
+        b.evals.alloc(2);
         std::vector<int32_t> caps(nproblems);
         for (int q = 0; q < nproblems; q++) caps[q] = mg_round_cap(b.grids[q]);
         TS_HIP(hipMemcpy(b.cap.p, caps.data(), nproblems * 4, hipMemcpyHostToDevice));
@@ -736,6 +751,10 @@ int ts_gridbatch_solve(ts_gridbatch *b, int mode, const volatile int32_t *cancel
         mg_solve(*b, mode, cancel);
         b->last_mode = mode;
         b->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        unsigned long long ev[2] = {0, 0};
+        TS_HIP(hipMemcpy(ev, b->evals.p, sizeof ev, hipMemcpyDeviceToHost));
+        b->updates[0] = (double)ev[0];
+        b->updates[1] = (double)ev[1];
         if (rounds_out) std::copy(b->rounds_host.begin(), b->rounds_host.end(), rounds_out);
         return TS_OK;
     });
@@ -752,6 +771,14 @@ int ts_gridbatch_read(ts_gridbatch *b, int problem, double *out)
         TS_HIP(hipMemcpy(out, b->T64.p + b->base[problem], N * 8, hipMemcpyDeviceToHost));
         return TS_OK;
     });
+}
+
+int ts_gridbatch_updates(ts_gridbatch *b, double *fp32, double *fp64)
+{
+    if (!b) return TS_EINVAL;
+    if (fp32) *fp32 = b->updates[0];
+    if (fp64) *fp64 = b->updates[1];
+    return TS_OK;
 }
 
 int ts_gridbatch_stats(ts_gridbatch *b, int64_t *nodes, double *seconds, double *sweep_seconds)

@@ -31,6 +31,8 @@ def _declare():
     L.ts_gridbatch_read.argtypes = [vp, i, vp]
     L.ts_gridbatch_stats.restype = i
     L.ts_gridbatch_stats.argtypes = [vp, vp, vp, vp]
+    L.ts_gridbatch_updates.restype = i
+    L.ts_gridbatch_updates.argtypes = [vp, vp, vp]
     L.ts_gridbatch_release.restype = None
     L.ts_gridbatch_release.argtypes = [vp]
     L._gridbatch_declared = True
@@ -90,7 +92,10 @@ class GridBatch:
     def stats(self):
         n, s, k = C.c_int64(), C.c_double(), C.c_double()
         check(_declare().ts_gridbatch_stats(self._h, C.byref(n), C.byref(s), C.byref(k)))
-        return {"nodes": n.value, "seconds": s.value, "sweep_seconds": k.value}
+        u32, u64 = C.c_double(), C.c_double()
+        check(_declare().ts_gridbatch_updates(self._h, C.byref(u32), C.byref(u64)))
+        return {"nodes": n.value, "seconds": s.value, "sweep_seconds": k.value, "updates_fp32": u32.value,
+                "updates_fp64": u64.value}
 
     def release(self):
         if getattr(self, "_h", None) is not None:
