@@ -52,6 +52,7 @@ struct SrvRequest {
     // tables
     ts_grid grid{};
     int models[2] = {-1, -1};
+    const SrvModel *model_ptr[2] = {nullptr, nullptr};  // taken under the server's lock at submission
     std::vector<int32_t> model_of_field, seed_idx;
     std::vector<int64_t> seed_off;
     std::vector<double> seed_time;
@@ -117,7 +118,7 @@ void run_request(ts_server &S, ts_context *w, SrvRequest &r)
 {
     char why[256] = {0};
     if (r.kind == 0) {
-        const SrvModel *mp[2] = {S.models[r.models[0]].get(), r.models[1] >= 0 ? S.models[r.models[1]].get() : nullptr};
+        const SrvModel *const *mp = r.model_ptr;
         const int nm = mp[1] ? 2 : 1;
         const double *d64[2] = {mp[0]->d64.p, mp[1] ? mp[1]->d64.p : nullptr};
         const float *d32[2] = {mp[0]->d32.p, mp[1] ? mp[1]->d32.p : nullptr};
@@ -258,19 +259,24 @@ int ts_server_submit_tables(ts_server *S, int32_t model_p, int32_t model_s, int 
     return guarded(reason, reason_len, [&]() -> int {
         std::unique_ptr<SrvRequest> r(new SrvRequest);
         const int nm = model_s >= 0 ? 2 : 1;
+        // the model pointers are taken under the lock (a concurrent add_model may reallocate the vector);
+        // the models themselves stay where they are until the server is destroyed
+        const SrvModel *mp = nullptr, *ms = nullptr;
         {
             std::lock_guard<std::mutex> lk(S->m);
             if (model_p < 0 || model_p >= (int)S->models.size() || model_s >= (int)S->models.size())
                 fail(TS_EINVAL, "no such model");
+            mp = S->models[model_p].get();
+            ms = nm == 2 ? S->models[model_s].get() : nullptr;
         }
-        const SrvModel *mp = S->models[model_p].get();
-        const SrvModel *ms = nm == 2 ? S->models[model_s].get() : nullptr;
         if (ms && (ms->grid.nx != mp->grid.nx || ms->grid.ny != mp->grid.ny || ms->grid.nz != mp->grid.nz))
             fail(TS_EINVAL, "the P and S models lie on different grids");
         r->kind = 0;
         r->grid = mp->grid;
         r->models[0] = model_p;
         r->models[1] = nm == 2 ? model_s : -1;
+        r->model_ptr[0] = mp;
+        r->model_ptr[1] = ms;
         r->mode = mode;
         // one field per source and phase (SphericalEikonal.PointSource seeds, :68-100)
         const Grid g = make_grid(mp->grid);
