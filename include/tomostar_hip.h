@@ -178,6 +178,29 @@ void ts_gridbatch_release(ts_gridbatch *b);
 int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *field_station, int nrays,
                   const int32_t *field_of_ray, const double *events, double step_km, ts_rays **out,
                   char *reason, int reason_len);
+/* The 1-D background of an extended forward domain (ForwardDomain.Background; VelocityModel1D,
+ * Model/VelocityModel1D.cs:8-45): count nodes of depth_km (not decreasing; a repeated depth is a step,
+ * a point on it takes the upper value), vp and vs (km/s, 0 < vs < vp), piecewise linear in depth,
+ * constant above the first node and below the last. */
+typedef struct ts_background {
+    int count;
+    const double *depth_km;
+    const double *vp;
+    const double *vs;
+} ts_background;
+/* ts_rays_trace on an extended forward domain (RayKernels.cs:62-64, 95): with background non-NULL,
+ * every kernel piece whose midpoint lies outside the inversion grid (SphericalGrid.Contains, tolerance
+ * 1e-6 in node units) is not attributed to the grid's edge nodes; its length times 1/v_bg(depth) of the
+ * arrival's phase (phase_of_ray[r]: 0 P, 1 S) is summed, in piece order, into the row's OutsideTime
+ * (RayTracing.Kernel, RayTracing.cs:126-147). background NULL: ts_rays_trace (phase_of_ray unused,
+ * may be NULL). The background is validated as VelocityModel1D.Validate (:47-56) does. */
+int ts_rays_trace_extended(ts_tableset *tables, const ts_grid *inversion, const double *field_station, int nrays,
+                           const int32_t *field_of_ray, const int32_t *phase_of_ray, const double *events,
+                           double step_km, const ts_background *background, ts_rays **out, char *reason,
+                           int reason_len);
+/* RayRow.OutsideTime (RayKernels.cs:18) per arrival, outside[nrays]: 0 for arrivals without a row and
+ * for rays traced without a background. */
+int ts_rays_outside_time(ts_rays *r, double *outside);
 /* Rows in arrival order: counts[nrays] (nodes per row, -1 = no row: stalled descent or unreached
  * event, as RayKernels skips it), row_start[nrays+1] (CSR over all arrivals, empty rows for the
  * skipped ones), nodes/lengths[row_start[nrays]], tcalc_grad[4*nrays] (TCalc, Ge, Gn, Gz) and
@@ -282,9 +305,10 @@ int ts_system_build(ts_context *ctx, const ts_grid *grid, const ts_system_opts *
 
 /* ---------------------------------------------------------------- Q (attenuation) tomography
  * QTomography.Predict (Attenuation/QTomography.cs:307-312) over rows of resident rays: per row r (arrival
- * ray_of_row[r]) the predicted t* = q0 * OutsideTime + sum L s q (rows inside the grid: no outside
- * part) and the path time sum L s (Prepare's Q0 estimate, :232-233), each summed in the row's node
- * order. slowness, q: [nodes of the rays' inversion grid]; q NULL = q0 at every node. */
+ * ray_of_row[r]) the predicted t* = q0 * OutsideTime + sum L s q and the path time OutsideTime +
+ * sum L s (Prepare's Q0 estimate, :231-233), each summed from the outside part on in the row's node
+ * order (OutsideTime is 0 for rays traced without a background). slowness, q: [nodes of the rays'
+ * inversion grid]; q NULL = q0 at every node. */
 int ts_q_predict(ts_rays *rays, int nrows, const int32_t *ray_of_row, const double *slowness, const double *q,
                  double q0, double *pred, double *travel_time, char *reason, int reason_len);
 

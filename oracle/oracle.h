@@ -74,6 +74,18 @@ void or_table_gradient(const or_griddef *g, const float *table, double lon, doub
                        double out[3]);
 
 /* --- rays (Forward/RayTracing.cs, Tomography/RayKernels.cs) -------------------------------- */
+/* A 1-D background model (Model/VelocityModel1D.cs): node depths (km, non-decreasing), Vp, Vs (km/s). */
+typedef struct {
+    int count;
+    const double *depth_km, *vp, *vs;
+} or_background;
+double or_background_velocity(const or_background *bg, int phase, double depth);
+int or_ray_kernel_bg(const or_griddef *inv, int npts, const double *plon, const double *plat, const double *pdep,
+                     const or_background *bg, int phase, int cap, int *nodes_out, double *len_out, double *outside);
+int or_ray_rows_batch_bg(const or_griddef *fwd, const float *const *tables, const double *stations,
+                         const or_griddef *inv, int nrays, const int *ray_table, const double *events,
+                         double step_km, const or_background *bg, const int *phase_of_ray, int cap_per_ray,
+                         int threads, int *counts, int *nodes, float *lens, double *tcalc_grad, double *outside);
 int or_backtrack(const or_griddef *g, const float *table, double st_lon, double st_lat, double st_depth,
                  double lon, double lat, double depth, double step_km, int cap,
                  double *plon, double *plat, double *pdep);
@@ -125,12 +137,12 @@ typedef struct {
 } or_qsystem_opts;
 
 void or_q_predict(int nrows, const int64_t *row_start, const int *row_nodes, const float *row_len,
-                  const double *slow, const double *q, double q0, double *pred, double *tt);
+                  const double *slow, const double *q, double q0, const double *outside, double *pred, double *tt);
 int or_build_qsystem(const or_griddef *g, const or_qsystem_opts *o, int nrows, const int64_t *row_start,
-                     const int *row_nodes, const float *row_len, const int *station, const double *tstar,
-                     const double *sigma, const double *slow, const double *q_old, const double *m_old,
-                     int64_t cap_rows, int64_t cap_nnz, int64_t *rs, int *ci, double *va, double *rhs,
-                     int64_t dims[4], double scales[2]);
+                     const int *row_nodes, const float *row_len, const double *outside, const int *station,
+                     const double *tstar, const double *sigma, const double *slow, const double *q_old,
+                     const double *m_old, int64_t cap_rows, int64_t cap_nnz, int64_t *rs, int *ci, double *va,
+                     double *rhs, int64_t dims[4], double scales[2]);
 
 #ifdef __cplusplus
 }

@@ -421,9 +421,45 @@ def ray_kernel(inv: GridDef, path, cap=100000):
     return nodes[:n].copy(), lens[:n].copy()
 
 
+class Background(C.Structure):
+    """or_background: a 1-D model (Model/VelocityModel1D.cs), node depths and Vp, Vs."""
+
+    _fields_ = [("count", C.c_int), ("depth_km", C.c_void_p), ("vp", C.c_void_p), ("vs", C.c_void_p)]
+
+
+def background(depth_km, vp, vs):
+    """(or_background, the arrays it points to: keep them alive while it is used)."""
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (depth_km, vp, vs)]
+    return Background(len(arrs[0]), arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data), arrs
+
+
+def background_velocity(bg, phase, depth):
+    L = lib()
+    L.or_background_velocity.argtypes = [C.POINTER(Background), C.c_int, C.c_double]
+    L.or_background_velocity.restype = C.c_double
+    return L.or_background_velocity(C.byref(bg[0]), phase, depth)
+
+
+def ray_kernel_bg(inv: GridDef, path, bg, phase, cap=100000):
+    """RayTracing.Kernel with a background: (nodes, lengths, outside time)."""
+    L = lib()
+    L.or_ray_kernel_bg.argtypes = [_G, C.c_int, _dp, _dp, _dp, C.c_void_p, C.c_int, C.c_int, _ip, _dp,
+                                   C.POINTER(C.c_double)]
+    lon, lat, dep = (np.ascontiguousarray(a, np.float64) for a in path)
+    nodes, lens = np.zeros(cap, np.int32), np.zeros(cap)
+    out = C.c_double()
+    n = L.or_ray_kernel_bg(C.byref(inv), len(lon), lon, lat, dep, C.addressof(bg[0]) if bg else None, phase, cap,
+                           nodes, lens, C.byref(out))
+    if n < 0:
+        raise ValueError("kernel longer than cap")
+    return nodes[:n].copy(), lens[:n].copy(), out.value
+
+
 def ray_rows_batch(fwd: GridDef, tables, stations, inv: GridDef, ray_table, events, step_km, cap_per_ray=4096,
-                   threads=0):
-    """RayKernels.Compute's rows for many arrivals: list of (nodes, float32 lengths, tcalc, ge, gn, gz) or None."""
+                   threads=0, bg=None, phase_of_ray=None):
+    """RayKernels.Compute's rows for many arrivals: list of (nodes, float32 lengths, tcalc, ge, gn, gz) or None;
+    with a background (bg from background(), phase_of_ray 0 P / 1 S), the rows' outside times as well:
+    (rows, outside[nrays])."""
     tabs = [np.ascontiguousarray(t, np.float32) for t in tables]
     ptrs = (C.c_void_p * len(tabs))(*[t.ctypes.data for t in tabs])
     st = np.ascontiguousarray(stations, np.float64).ravel()
@@ -434,8 +470,14 @@ def ray_rows_batch(fwd: GridDef, tables, stations, inv: GridDef, ray_table, even
     nodes = np.zeros(nr * cap_per_ray, np.int32)
     lens = np.zeros(nr * cap_per_ray, np.float32)
     tg = np.zeros(4 * nr)
-    lib().or_ray_rows_batch(C.byref(fwd), ptrs, st, C.byref(inv), nr, rt, ev, step_km, cap_per_ray, threads,
-                            counts, nodes, lens, tg)
+    outside = np.zeros(max(1, nr))
+    L = lib()
+    L.or_ray_rows_batch_bg.argtypes = [_G, C.POINTER(C.c_void_p), _dp, _G, C.c_int, _ip, _dp, C.c_double,
+                                       C.c_void_p, C.c_void_p, C.c_int, C.c_int, _ip, _ip, _fp, _dp, _dp]
+    ph = None if phase_of_ray is None else np.ascontiguousarray(phase_of_ray, np.int32)
+    L.or_ray_rows_batch_bg(C.byref(fwd), ptrs, st, C.byref(inv), nr, rt, ev, step_km,
+                           C.addressof(bg[0]) if bg else None, None if ph is None else ph.ctypes.data,
+                           cap_per_ray, threads, counts, nodes, lens, tg, outside)
     out = []
     for r in range(nr):
         c = counts[r]
@@ -444,7 +486,7 @@ def ray_rows_batch(fwd: GridDef, tables, stations, inv: GridDef, ray_table, even
         else:
             a = r * cap_per_ray
             out.append((nodes[a:a + c].copy(), lens[a:a + c].copy(), *tg[4 * r:4 * r + 4]))
-    return out
+    return (out, outside[:nr].copy()) if bg else out
 
 
 class SystemOpts(C.Structure):
@@ -510,9 +552,10 @@ def _declare_q(L):
     if getattr(L, "_q_declared", False):
         return
     L.or_q_predict.restype = None
-    L.or_q_predict.argtypes = [C.c_int, _lp, _ip, _fp, _dp, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+    L.or_q_predict.argtypes = [C.c_int, _lp, _ip, _fp, _dp, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                               C.c_void_p]
     L.or_build_qsystem.restype = C.c_int
-    L.or_build_qsystem.argtypes = [_G, C.POINTER(QSystemOpts), C.c_int, _lp, _ip, _fp, _ip, _dp, _dp, _dp,
+    L.or_build_qsystem.argtypes = [_G, C.POINTER(QSystemOpts), C.c_int, _lp, _ip, _fp, C.c_void_p, _ip, _dp, _dp, _dp,
                                    C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, _lp, _ip, _dp, _dp, _lp, _dp]
     L._q_declared = True
 
@@ -521,22 +564,23 @@ def _vp(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-def q_predict(row_start, nodes, lens, slow, q, q0):
-    """QTomography.Predict (QTomography.cs:307-312) and the path time sum L s (:232-233) of every row
-    (rows inside the grid); q None: q0 at every node. Returns (pred, tt)."""
+def q_predict(row_start, nodes, lens, slow, q, q0, outside=None):
+    """QTomography.Predict (QTomography.cs:307-312) and the path time OutsideTime + sum L s (:231-233) of
+    every row; outside: the rows' OutsideTime (None: 0); q None: q0 at every node. Returns (pred, tt)."""
     L = lib()
     _declare_q(L)
     rs = np.ascontiguousarray(row_start, np.int64)
     nr = rs.size - 1
     pred, tt = np.zeros(max(1, nr)), np.zeros(max(1, nr))
     qa = None if q is None else np.ascontiguousarray(q, np.float64)
+    oa = None if outside is None else np.ascontiguousarray(outside, np.float64)
     L.or_q_predict(nr, rs, np.ascontiguousarray(nodes, np.int32), np.ascontiguousarray(lens, np.float32),
-                   np.ascontiguousarray(slow, np.float64), _vp(qa), float(q0), _vp(pred), _vp(tt))
+                   np.ascontiguousarray(slow, np.float64), _vp(qa), float(q0), _vp(oa), _vp(pred), _vp(tt))
     return pred[:nr], tt[:nr]
 
 
 def build_qsystem(g: GridDef, opts: dict, row_start, nodes, lens, station, tstar, sigma, slow, q_old=None, m_old=None,
-                  scales=None):
+                  scales=None, outside=None):
     """QTomography.SolveOnce's system restated (QTomography.cs:318-392, plain node grid); the rows are the
     used ray rows in order. Returns (row_start, column, values, rhs, dims, scales) with dims = (rows,
     cols, nnz, data rows) and scales = (model, station)."""
@@ -559,7 +603,8 @@ def build_qsystem(g: GridDef, opts: dict, row_start, nodes, lens, station, tstar
     qo = None if q_old is None else np.ascontiguousarray(q_old, np.float64)
     mo = None if m_old is None else np.ascontiguousarray(m_old, np.float64)
     st = np.ascontiguousarray(station if len(station) else np.zeros(1), np.int32)
-    rc = L.or_build_qsystem(C.byref(g), C.byref(o), nr, rs_in, nodes, np.ascontiguousarray(lens, np.float32), st,
+    oa = None if outside is None else np.ascontiguousarray(outside, np.float64)
+    rc = L.or_build_qsystem(C.byref(g), C.byref(o), nr, rs_in, nodes, np.ascontiguousarray(lens, np.float32), _vp(oa), st,
                             np.ascontiguousarray(tstar, np.float64) if nr else np.zeros(1),
                             np.ascontiguousarray(sigma, np.float64) if nr else np.ones(1),
                             np.ascontiguousarray(slow, np.float64), _vp(qo), _vp(mo), cap_rows, cap_nnz, rs, ci, va,

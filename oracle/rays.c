@@ -140,12 +140,33 @@ static int contrib_cmp(const void *a, const void *b)
     return x->order < y->order ? -1 : (x->order > y->order);
 }
 
-/* RayTracing.cs:126-159 Kernel without a background: kernel[node] = sum of dl * w over the pieces,
- * each node accumulated in the order the pieces reach it (the Dictionary's old + v), zero weights
- * skipped. Output sorted by node (RayKernels.cs:96). Returns the node count, or -2 beyond cap. */
-int or_ray_kernel(const or_griddef *inv, int npts, const double *plon, const double *plat,
-                  const double *pdep, int cap, int *nodes_out, double *len_out)
+/* VelocityModel1D.Evaluate (Model/VelocityModel1D.cs:31-45): piecewise linear in depth; at or above the
+ * first node its value; a repeated depth is a step (a point on it takes the upper value); below the
+ * last node its value. phase 0 Vp, 1 Vs. */
+double or_background_velocity(const or_background *bg, int phase, double depth)
 {
+    const double *v = phase ? bg->vs : bg->vp;
+    if (depth <= bg->depth_km[0]) return v[0];
+    for (int i = 1; i < bg->count; i++) {
+        const double a = bg->depth_km[i - 1], b = bg->depth_km[i];
+        if (depth > b) continue;
+        if (b - a <= 1e-12) continue;
+        const double t = (depth - a) / (b - a);
+        return v[i - 1] + t * (v[i] - v[i - 1]);
+    }
+    return v[bg->count - 1];
+}
+
+/* RayTracing.cs:126-159 Kernel: kernel[node] = sum of dl * w over the pieces, each node accumulated in
+ * the order the pieces reach it (the Dictionary's old + v), zero weights skipped. With a background
+ * (bg != NULL, the extended forward domain of RayKernels.cs:62-64), a piece whose midpoint lies outside
+ * the inversion grid (SphericalGrid.Contains, tolerance 1e-6 in node units, SphericalGrid.cs:186-192)
+ * adds dl / v_bg(phase, depth) to *outside instead (:141-145). Output sorted by node (RayKernels.cs:96).
+ * Returns the node count, or -2 beyond cap. */
+int or_ray_kernel_bg(const or_griddef *inv, int npts, const double *plon, const double *plat, const double *pdep,
+                     const or_background *bg, int phase, int cap, int *nodes_out, double *len_out, double *outside)
+{
+    double outside_time = 0;
     double h = 0.5 * or_min_spacing_km(inv);
     int ncap = 1024, nc = 0;
     contrib_t *c = malloc(sizeof(contrib_t) * ncap);
@@ -165,6 +186,15 @@ int or_ray_kernel(const or_griddef *inv, int npts, const double *plon, const dou
             double mid[3], fr[3];
             or_from_cartesian(m, mid);
             or_fractional(inv, mid[0], mid[1], mid[2], fr);
+            if (bg) {
+                const double tol = 1e-6;
+                const int in = fr[0] >= -tol && fr[0] <= inv->nx - 1 + tol && fr[1] >= -tol &&
+                               fr[1] <= inv->ny - 1 + tol && fr[2] >= -tol && fr[2] <= inv->nz - 1 + tol;
+                if (!in) {
+                    outside_time += dl * (1.0 / or_background_velocity(bg, phase, mid[2]));
+                    continue;
+                }
+            }
             int nodes[8];
             double w[8];
             or_trilinear_weights_fractional(inv, fr[0], fr[1], fr[2], nodes, w);
@@ -192,15 +222,23 @@ int or_ray_kernel(const or_griddef *inv, int npts, const double *plon, const dou
         q = r;
     }
     free(c);
+    if (outside) *outside = outside_time;
     return count;
+}
+
+int or_ray_kernel(const or_griddef *inv, int npts, const double *plon, const double *plat,
+                  const double *pdep, int cap, int *nodes_out, double *len_out)
+{
+    return or_ray_kernel_bg(inv, npts, plon, plat, pdep, NULL, 0, cap, nodes_out, len_out, NULL);
 }
 
 /* One arrival of RayKernels.Compute (RayKernels.cs:77-105): back-trace on the station's table, the
  * time and gradient at the event, the kernel on the inversion grid with float32 lengths.
  * tcalc_grad = {TCalc, Ge, Gn, Gz}. Returns the node count, -1 when the arrival yields no row. */
-int or_ray_row(const or_griddef *fwd, const float *table, double st_lon, double st_lat, double st_depth,
-               const or_griddef *inv, double ev_lon, double ev_lat, double ev_depth, double step_km,
-               int cap, int *nodes_out, float *len_out, double tcalc_grad[4], int *npath_out)
+static int ray_row_bg(const or_griddef *fwd, const float *table, double st_lon, double st_lat, double st_depth,
+                      const or_griddef *inv, double ev_lon, double ev_lat, double ev_depth, double step_km,
+                      const or_background *bg, int phase, int cap, int *nodes_out, float *len_out,
+                      double tcalc_grad[4], int *npath_out, double *outside)
 {
     int pcap = 20 * ((int)ceil(2 * 6371.0 / step_km) + 40);
     double *pl = malloc(sizeof(double) * pcap * 3);
@@ -213,7 +251,7 @@ int or_ray_row(const or_griddef *fwd, const float *table, double st_lon, double 
     or_table_gradient(fwd, table, ev_lon, ev_lat, ev_depth, gr);
     if (!isfinite(tcalc)) { free(pl); return -1; }
     double *len = malloc(sizeof(double) * (cap > 0 ? cap : 1));
-    int n = or_ray_kernel(inv, np, pl, pl + pcap, pl + 2 * pcap, cap, nodes_out, len);
+    int n = or_ray_kernel_bg(inv, np, pl, pl + pcap, pl + 2 * pcap, bg, phase, cap, nodes_out, len, outside);
     for (int q = 0; q < n; q++) len_out[q] = (float)len[q];
     tcalc_grad[0] = tcalc;
     tcalc_grad[1] = gr[0];
@@ -224,13 +262,21 @@ int or_ray_row(const or_griddef *fwd, const float *table, double st_lon, double 
     return n;
 }
 
+int or_ray_row(const or_griddef *fwd, const float *table, double st_lon, double st_lat, double st_depth,
+               const or_griddef *inv, double ev_lon, double ev_lat, double ev_depth, double step_km,
+               int cap, int *nodes_out, float *len_out, double tcalc_grad[4], int *npath_out)
+{
+    return ray_row_bg(fwd, table, st_lon, st_lat, st_depth, inv, ev_lon, ev_lat, ev_depth, step_km, NULL, 0, cap,
+                      nodes_out, len_out, tcalc_grad, npath_out, NULL);
+}
+
 /* Many arrivals in parallel over the host cores (Parallel.ForEach, RayKernels.cs:67): ray r uses
  * table ray_table[r] (stations[3*t..] its station) and event events[3*r..]; its row goes to
  * nodes/lens[r*cap_per_ray ...], counts[r] = node count or -1. */
-int or_ray_rows_batch(const or_griddef *fwd, const float *const *tables, const double *stations,
-                      const or_griddef *inv, int nrays, const int *ray_table, const double *events,
-                      double step_km, int cap_per_ray, int threads,
-                      int *counts, int *nodes, float *lens, double *tcalc_grad)
+int or_ray_rows_batch_bg(const or_griddef *fwd, const float *const *tables, const double *stations,
+                         const or_griddef *inv, int nrays, const int *ray_table, const double *events,
+                         double step_km, const or_background *bg, const int *phase_of_ray, int cap_per_ray,
+                         int threads, int *counts, int *nodes, float *lens, double *tcalc_grad, double *outside)
 {
 #ifdef _OPENMP
     if (threads <= 0) threads = omp_get_max_threads();
@@ -238,11 +284,22 @@ int or_ray_rows_batch(const or_griddef *fwd, const float *const *tables, const d
 #endif
     for (int r = 0; r < nrays; r++) {
         int t = ray_table[r];
-        counts[r] = or_ray_row(fwd, tables[t], stations[3 * t], stations[3 * t + 1], stations[3 * t + 2], inv,
-                               events[3 * r], events[3 * r + 1], events[3 * r + 2], step_km, cap_per_ray,
-                               nodes + (size_t)r * cap_per_ray, lens + (size_t)r * cap_per_ray,
-                               tcalc_grad + 4 * (size_t)r, NULL);
+        double out_t = 0;
+        counts[r] = ray_row_bg(fwd, tables[t], stations[3 * t], stations[3 * t + 1], stations[3 * t + 2], inv,
+                               events[3 * r], events[3 * r + 1], events[3 * r + 2], step_km, bg,
+                               phase_of_ray ? phase_of_ray[r] : 0, cap_per_ray, nodes + (size_t)r * cap_per_ray,
+                               lens + (size_t)r * cap_per_ray, tcalc_grad + 4 * (size_t)r, NULL, &out_t);
+        if (outside) outside[r] = out_t;
     }
     (void)threads;
     return 0;
+}
+
+int or_ray_rows_batch(const or_griddef *fwd, const float *const *tables, const double *stations,
+                      const or_griddef *inv, int nrays, const int *ray_table, const double *events,
+                      double step_km, int cap_per_ray, int threads,
+                      int *counts, int *nodes, float *lens, double *tcalc_grad)
+{
+    return or_ray_rows_batch_bg(fwd, tables, stations, inv, nrays, ray_table, events, step_km, NULL, NULL, cap_per_ray,
+                                threads, counts, nodes, lens, tcalc_grad, NULL);
 }

@@ -239,13 +239,15 @@ int or_build_system(const or_griddef *g, const or_system_opts *o, int nrows, con
 
 /* ---------------------------------------------------------------- Q tomography
  * QTomography.Predict (Attenuation/QTomography.cs:307-312) and the path time of Prepare's Q0 estimate
- * (:232-233): per row t* = q0 * OutsideTime + sum L s q and T = OutsideTime + sum L s, in the row's
- * order. The rows lie inside the grid (OutsideTime 0). q NULL: q0 at every node. */
+ * (:231-233): per row t* = q0 * OutsideTime + sum L s q and T = OutsideTime + sum L s, in the row's
+ * order. outside[r]: the row's OutsideTime (NULL: 0, rows inside the grid). q NULL: q0 at every node. */
 void or_q_predict(int nrows, const int64_t *row_start, const int *row_nodes, const float *row_len,
-                  const double *slow, const double *q, double q0, double *pred, double *tt)
+                  const double *slow, const double *q, double q0, const double *outside, double *pred, double *tt)
 {
     for (int r = 0; r < nrows; r++) {
-        double t = q0 * 0.0, T = 0.0;
+        /* Predict starts from q0 * OutsideTime (:309), the Q0 estimate's path time from OutsideTime (:231) */
+        const double out = outside ? outside[r] : 0.0;
+        double t = q0 * out, T = out;
         for (int64_t k = row_start[r]; k < row_start[r + 1]; k++) {
             int nd = row_nodes[k];
             t += (double)row_len[k] * slow[nd] * (q ? q[nd] : q0);
@@ -266,10 +268,10 @@ void or_q_predict(int nrows, const int64_t *row_start, const int *row_nodes, con
  * scales[2] in/out (model, station). Returns 0, or -1 when the output arrays are too small;
  * dims = {rows, cols, nnz, data rows}. */
 int or_build_qsystem(const or_griddef *g, const or_qsystem_opts *o, int nrows, const int64_t *row_start,
-                     const int *row_nodes, const float *row_len, const int *station, const double *tstar,
-                     const double *sigma, const double *slow, const double *q_old, const double *m_old,
-                     int64_t cap_rows, int64_t cap_nnz, int64_t *rs, int *ci, double *va, double *rhs,
-                     int64_t dims[4], double scales[2])
+                     const int *row_nodes, const float *row_len, const double *outside, const int *station,
+                     const double *tstar, const double *sigma, const double *slow, const double *q_old,
+                     const double *m_old, int64_t cap_rows, int64_t cap_nnz, int64_t *rs, int *ci, double *va,
+                     double *rhs, int64_t dims[4], double scales[2])
 {
     const int n = g->nx * g->ny * g->nz;
     const int off_st = o->station_terms ? n : -1;
@@ -294,11 +296,11 @@ int or_build_qsystem(const or_griddef *g, const or_qsystem_opts *o, int nrows, c
         double d;
         if (!q_old) {
             double p = 0;
-            or_q_predict(1, row_start + r, row_nodes, row_len, slow, NULL, o->q0, &p, NULL);
+            or_q_predict(1, row_start + r, row_nodes, row_len, slow, NULL, o->q0, outside ? outside + r : NULL, &p, NULL);
             d = tstar[r] - p;
         } else {
             double p = 0;
-            or_q_predict(1, row_start + r, row_nodes, row_len, slow, q_old, o->q0, &p, NULL);
+            or_q_predict(1, row_start + r, row_nodes, row_len, slow, q_old, o->q0, outside ? outside + r : NULL, &p, NULL);
             d = tstar[r] - p;
             for (int k = 0; k < len; k++) d += v[k] * m_old[c[k]];
         }

@@ -52,7 +52,8 @@ def _compare(batch, ref):
         assert (dev.ge, dev.gn, dev.gz) == (ge, gn, gz)
         if np.array_equal(nodes, dev.nodes):
             same_cells += 1
-            worst_len = max(worst_len, float(np.max(np.abs(dev.length - lens) / np.maximum(lens, 1e-30))))
+            if nodes.size:  # a ray wholly outside an inversion grid has an empty row
+                worst_len = max(worst_len, float(np.max(np.abs(dev.length - lens) / np.maximum(lens, 1e-30))))
         # path length and the travel time along the ray: within 1e-4 whatever the piece splits
         total_ref, total_dev = float(np.sum(lens, dtype=np.float64)), float(np.sum(dev.length, dtype=np.float64))
         assert abs(total_dev - total_ref) <= 1e-4 * total_ref
@@ -162,3 +163,97 @@ def test_unreached_events_give_no_row(device):
     batch = Ry.RayBatch(ts, g, np.array([[13.05, 43.1, 0.0]]), np.zeros(2, np.int32), ev)
     assert batch.counts[0] > 0 and batch.counts[1] == -1
     assert len(batch.rows()) == 1
+
+
+def test_extended_domain_rows_and_outside_times(device, oracle):
+    """An extended forward domain (ForwardDomain.Covering, Forward/ForwardDomain.cs:48-80; RayKernels.cs:62-64,
+    95): tables on a forward grid reaching beyond the inversion grid on every side but the top, a 1-D
+    background with a step at 10 km outside it; stations and events inside and outside the inversion
+    grid. The pieces outside go to each row's OutsideTime (RayTracing.cs:141-145): the device's cell
+    lists match the oracle's (>= 99.9 %), and on those rows the OutsideTime is the oracle's to the bit
+    (the same pieces, summed in the same order with the same correctly rounded geometry). Q's Predict and
+    path time then include it (QTomography.cs:231, :309), equal to the oracle's to the bit."""
+    from tomostar_amd import eikonal as E
+    from tomostar_amd import qtomography as QT
+    from tomostar_amd import rays as Ry
+    from tomostar_amd import workloads as W
+    from tomostar_amd.grid import GridDefinition
+
+    inv = GridDefinition.from_spacing(13.2, 13.56, 43.1, 43.37, 0, 20, 2, 2)
+    fwd = GridDefinition.from_spacing(13.0, 13.74, 43.0, 43.54, 0, 30, 1, 1)
+    depth, vp1, vs1 = [0.0, 10.0, 10.0, 35.0], [5.0, 6.0, 6.5, 7.0], [2.9, 3.5, 3.75, 4.05]
+    bgv = [np.interp(fwd.depth(), depth[2:], v[2:]) for v in (vp1, vs1)]
+    for v, vv in zip(bgv, (vp1, vs1)):
+        up = fwd.depth() <= 10.0
+        v[up] = np.interp(fwd.depth()[up], depth[:2], vv[:2])
+    lon, lat, dep = fwd.lon(), fwd.lat(), fwd.depth()
+    inside = ((lon[None, None, :] >= inv.min_lon) & (lon[None, None, :] <= inv.max_lon) &
+              (lat[None, :, None] >= inv.min_lat) & (lat[None, :, None] <= inv.max_lat) & (dep[:, None, None] <= 20))
+    pert = 1 + W.checkerboard(fwd, 8.0, 8.0, 0.05) * inside
+    vp = np.ascontiguousarray(bgv[0][:, None, None] * pert)
+    vs = np.ascontiguousarray(bgv[1][:, None, None] * pert)
+    rng = np.random.default_rng(12)
+    st = np.concatenate([np.stack([rng.uniform(13.22, 13.54, 10), rng.uniform(43.12, 43.35, 10), np.zeros(10)], 1),
+                         np.array([[13.05, 43.5, 0.0], [13.7, 43.02, 0.0]])])
+    ev = np.stack([rng.uniform(13.03, 13.71, 60), rng.uniform(43.03, 43.51, 60), rng.uniform(2, 28, 60)], 1)
+    wl = W.Workload("extended", fwd, vp, vs, st, ev)
+    seeds, mof = W.station_seed_sets(wl, E.point_source)
+    ts = E.TableSet(device, fwd, [wl.slowness_p, wl.slowness_s], seeds, mof)
+    assert ts.solve("fast_fp32")[0] == 0
+    field, events = _arrivals(wl)
+    phase = (field % 2).astype(np.int32)
+    stations = np.repeat(wl.stations, 2, axis=0)
+    step = Ry.ray_step(fwd, inv)
+    batch = Ry.RayBatch(ts, inv, stations, field, events, background=(depth, vp1, vs1), phase_of_ray=phase)
+    bg = oracle.background(depth, vp1, vs1)
+    ref, ref_out = oracle.ray_rows_batch(_og(oracle, fwd), list(ts.read_f32()), stations, _og(oracle, inv), field,
+                                         events, step, bg=bg, phase_of_ray=phase)
+    compared, same, worst = _compare(batch, ref)
+    exact = sum(1 for r, rr in enumerate(ref) if rr is not None and np.array_equal(rr[0], batch.row(r).nodes)
+                and batch.outside_time[r] == ref_out[r])
+    crossing = int(np.count_nonzero(ref_out > 0))
+    empty = sum(1 for rr in ref if rr is not None and rr[0].size == 0)
+    print(f"extended domain: {same}/{compared} identical cell lists, {exact} with the same OutsideTime to the bit; "
+          f"{crossing} rays leave the inversion grid, {empty} lie wholly outside it")
+    assert compared > 0.99 * len(ref) and crossing > 0.3 * compared and empty > 0
+    assert same >= 0.999 * compared and exact == same and worst <= 1e-4
+    ok = np.array([rr is not None for rr in ref])
+    assert np.all(np.abs(batch.outside_time[ok] - ref_out[ok]) <= 1e-4 * np.maximum(ref_out[ok], 1e-3))
+    assert np.all(batch.outside_time[~ok] == 0)
+    # without a background the same rays keep every piece on the grid: no OutsideTime
+    plain = Ry.RayBatch(ts, inv, stations, field, events, host_rows=False)
+    assert np.all(plain.outside_time == 0)
+    plain.release()
+    # Q: Predict and the path time start from the OutsideTime
+    rows = np.flatnonzero(batch.counts >= 0).astype(np.int32)
+    rows = rows[phase[rows] == 1]
+    s_inv = 1 / (3.5 + 0.02 * inv.depth())
+    s_inv = np.ascontiguousarray(np.broadcast_to(s_inv[:, None, None], inv.shape)).ravel()
+    q = np.full(inv.count, 1 / 250) * (1 + 0.1 * rng.standard_normal(inv.count))
+    pred, tt = QT.q_predict(batch, rows, s_inv, q, 1 / 300)
+    rs = np.concatenate([[0], np.cumsum(batch.counts[rows])]).astype(np.int64)
+    nd = np.concatenate([batch.row(r).nodes for r in rows])
+    ln = np.concatenate([batch.row(r).length for r in rows])
+    opred, ott = oracle.q_predict(rs, nd, ln, s_inv, q, 1 / 300, outside=batch.outside_time[rows])
+    assert np.array_equal(pred, opred) and np.array_equal(tt, ott)
+    assert np.all(tt[batch.outside_time[rows] > 0] > 0)
+    # SolveOnce's data right-hand side t* - Predict (:378) with the OutsideTime, fixed scales
+    from tomostar_amd.system import System
+    ns = len(wl.stations)
+    station = (field[rows] // 2).astype(np.int32)
+    tstar = opred * (1 + 0.05 * rng.standard_normal(rows.size))
+    sigma = np.full(rows.size, 0.01)
+    st_ = QT.QTomographySettings()
+    sysd = QT.build_qsystem(device, inv, st_, batch, rows, station, tstar, sigma, ns, s_inv, 1 / 300, st_.damping,
+                            st_.smoothing, None, None, (0.5, 2.0))
+    drs, dci, dva = System(sysd.matrix, sysd.rhs).read()
+    opts = dict(station_terms=1, nstations=ns, damping=st_.damping, smoothing=st_.smoothing,
+                vertical_weight=st_.vertical_weight, damping_station=st_.damping_station, q0=1 / 300)
+    ors, oci, ova, orhs, dims, _ = oracle.build_qsystem(_og(oracle, inv), opts, rs, nd, ln, station, tstar, sigma,
+                                                        s_inv, None, None, (0.5, 2.0),
+                                                        outside=batch.outside_time[rows])
+    assert np.array_equal(drs, ors) and np.array_equal(dci, oci) and np.array_equal(dva, ova)
+    assert np.array_equal(sysd.rhs, orhs)
+    sysd.matrix.release()
+    batch.release()
+    ts.release()

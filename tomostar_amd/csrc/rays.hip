@@ -90,7 +90,24 @@ struct RayJob {
     int field;        // table of the arrival
     int64_t path_at;  // first path point slot
     int path_cap;     // points that fit
+    int phase;        // 0 P, 1 S: the background velocity of the pieces outside the inversion grid
 };
+
+// VelocityModel1D.Evaluate (Model/VelocityModel1D.cs:31-45): piecewise linear in depth; at or above the
+// first node its value; a repeated depth is a step (a point on it takes the upper value); below the last
+// node its value.
+__device__ inline double background_velocity(const double *depth, const double *v, int count, double d)
+{
+    if (d <= depth[0]) return v[0];
+    for (int i = 1; i < count; i++) {
+        const double a = depth[i - 1], b = depth[i];
+        if (d > b) continue;
+        if (b - a <= 1e-12) continue;
+        const double t = (d - a) / (b - a);
+        return v[i - 1] + t * (v[i] - v[i - 1]);
+    }
+    return v[count - 1];
+}
 
 // RayTracing.Backtrack (RayTracing.cs:73-111). path: 3 doubles (lon, lat, depth) per point.
 // npts[ray] = points, -1 no ray (stalled / non-finite), -2 path longer than its slot.
@@ -182,6 +199,10 @@ struct RowsArgs {
     float *row_len;
     int64_t row_cap;
     int32_t *row_count;                // per job: nodes, -1 no row, -3 too many nodes
+    // the extended forward domain's background (bg_count 0: none, every piece goes to the grid's nodes)
+    int bg_count;
+    const double *bg_depth, *bg_vp, *bg_vs;
+    double *row_outside;               // per job: the time of the pieces outside the inversion grid
 };
 
 template <int kHashSlots>
@@ -196,6 +217,7 @@ __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
     __shared__ int32_t cnode[kChunkSeg * kMaxPiecesSeg * 8];
     __shared__ double cval[kChunkSeg * kMaxPiecesSeg * 8];
     __shared__ int32_t pstart[kChunkSeg + 1];
+    __shared__ double cout[kChunkSeg * kMaxPiecesSeg];  // a piece's time outside the grid (0: inside)
     __shared__ int overflow;
     const int lane = threadIdx.x;
     if ((int)blockIdx.x >= A.nblocks) return;
@@ -204,9 +226,13 @@ __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
     const RayJob job = A.jobs[q];
     const int np = A.npts[job.ray];
     if (np < 0) {
-        if (lane == 0) A.row_count[slot] = -1;
+        if (lane == 0) {
+            A.row_count[slot] = -1;
+            if (A.row_outside) A.row_outside[slot] = 0.0;
+        }
         return;
     }
+    double outside = 0.0;  // lane 0's running sum, in piece order (RayTracing.cs:143)
     for (int s = lane; s < kHashSlots; s += 64) hkey[s] = -1;
     if (lane == 0) overflow = 0;
     __syncthreads();
@@ -250,6 +276,18 @@ __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
                 const GeoPoint mid = from_cartesian(vadd(a, vscale(d, f)));
                 double fx, fy, fz;
                 fractional(g, mid.lon, mid.lat, mid.depth, fx, fy, fz);
+                // SphericalGrid.Contains(mid, 1e-6) (SphericalGrid.cs:186-192) when a background is given
+                const double tol = 1e-6;
+                const bool in = A.bg_count == 0 || (fx >= -tol && fx <= g.nx - 1 + tol && fy >= -tol &&
+                                                    fy <= g.ny - 1 + tol && fz >= -tol && fz <= g.nz - 1 + tol);
+                if (!in) {
+                    const double v = background_velocity(A.bg_depth, job.phase ? A.bg_vs : A.bg_vp, A.bg_count, mid.depth);
+                    cout[pi] = dl * (1.0 / v);
+#pragma unroll
+                    for (int c = 0; c < 8; c++) cnode[pi * 8 + c] = -1;
+                    continue;
+                }
+                cout[pi] = 0.0;
                 int nodes[8];
                 double w[8];
                 trilinear_weights(g, fx, fy, fz, nodes, w);
@@ -264,6 +302,7 @@ __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
             const int c = lane & 7;
             const int sub = lane >> 3;
             for (int pi = 0; pi < limit; pi++) {
+                if (lane == 0) outside += cout[pi];
                 if (sub == 0) {
                     const int node = cnode[pi * 8 + c];
                     if (node >= 0) {
@@ -304,6 +343,7 @@ __global__ __launch_bounds__(64) void ray_rows(RowsArgs A)
         count += __popcll(occ);
         __syncthreads();
     }
+    if (lane == 0 && A.row_outside) A.row_outside[slot] = outside;
     if (count > A.row_cap || overflow) {
         if (lane == 0) A.row_count[slot] = -3;
         return;
@@ -373,17 +413,33 @@ __global__ void ray_move(const int32_t *src_nodes, const float *src_len, const i
 
 extern "C" {
 
-int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *field_station, int nrays,
-                  const int32_t *field_of_ray, const double *events, double step_km, ts_rays **out, char *reason,
-                  int reason_len)
+int ts_rays_trace_extended(ts_tableset *tables, const ts_grid *inversion, const double *field_station, int nrays,
+                           const int32_t *field_of_ray, const int32_t *phase_of_ray, const double *events,
+                           double step_km, const ts_background *background, ts_rays **out, char *reason,
+                           int reason_len)
 {
     if (out) *out = nullptr;
     if (!tables || !inversion || !field_station || nrays < 0 || (nrays > 0 && (!field_of_ray || !events)) || !out ||
-        !(step_km > 0) || !grid_valid(*inversion)) {
+        !(step_km > 0) || !grid_valid(*inversion) ||
+        (background && (background->count < 1 || !background->depth_km || !background->vp || !background->vs ||
+                        (nrays > 0 && !phase_of_ray)))) {
         set_reason(reason, reason_len, "invalid argument");
         return TS_EINVAL;
     }
     return guarded(reason, reason_len, [&]() -> int {
+        // VelocityModel1D.Validate (Model/VelocityModel1D.cs:47-56)
+        const int nbg = background ? background->count : 0;
+        for (int i = 0; i < nbg; i++) {
+            const double vp = background->vp[i], vs = background->vs[i], z = background->depth_km[i];
+            if (!(vp > 0) || !(vs > 0) || !std::isfinite(vp) || !std::isfinite(vs))
+                fail(TS_EINVAL, "background node %d: velocities must be positive", i);
+            if (vs >= vp) fail(TS_EINVAL, "background node %d: Vs must be smaller than Vp", i);
+            if (!std::isfinite(z) || (i > 0 && z < background->depth_km[i - 1]))
+                fail(TS_EINVAL, "background node %d: depths must not decrease", i);
+        }
+        if (nbg > 0)
+            for (int r = 0; r < nrays; r++)
+                if (phase_of_ray[r] != 0 && phase_of_ray[r] != 1) fail(TS_EINVAL, "arrival %d: phase %d", r, phase_of_ray[r]);
         ts_context *ctx = tableset_ctx(tables);
         const float *tab = tableset_tables(tables);
         if (!tab) fail(TS_EINVAL, "the table set has not been solved");
@@ -401,6 +457,14 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
         R->counts.assign(nrays, -1);
         R->tcalc_grad.assign(4 * (size_t)nrays, 0.0);
         R->path_points.assign(nrays, 0);
+        R->outside.assign(nrays, 0.0);
+        R->extended = nbg > 0;
+        dbuf<double> d_bg(3 * (size_t)std::max(1, nbg));
+        if (nbg > 0) {
+            TS_HIP(hipMemcpyAsync(d_bg.p, background->depth_km, nbg * 8, hipMemcpyHostToDevice, st));
+            TS_HIP(hipMemcpyAsync(d_bg.p + nbg, background->vp, nbg * 8, hipMemcpyHostToDevice, st));
+            TS_HIP(hipMemcpyAsync(d_bg.p + 2 * nbg, background->vs, nbg * 8, hipMemcpyHostToDevice, st));
+        }
         const double h = 0.5 * min_spacing_km(inv);
         // arrivals ordered by table, then by event position, so a wavefront shares a table and a region
         std::vector<int> order(nrays);
@@ -455,11 +519,15 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
             const int64_t cap = kSlotsSmall / 2;
             dbuf<int32_t> rn((size_t)nb * cap), rc(nb);
             dbuf<float> rl((size_t)nb * cap);
-            RowsArgs A{inv, h, d_path.p, d_jobs.p, d_npts.p, nullptr, nb, rn.p, rl.p, cap, rc.p};
+            dbuf<double> ro(nb);
+            RowsArgs A{inv, h, d_path.p, d_jobs.p, d_npts.p, nullptr, nb, rn.p, rl.p, cap, rc.p,
+                       nbg, d_bg.p, d_bg.p + nbg, d_bg.p + 2 * nbg, ro.p};
             ray_rows<kSlotsSmall><<<nb, 64, 0, st>>>(A);
             TS_HIP(hipGetLastError());
             std::vector<int32_t> cnt(nb);
+            std::vector<double> outs(nb);
             TS_HIP(hipMemcpyAsync(cnt.data(), rc.p, nb * 4, hipMemcpyDeviceToHost, st));
+            TS_HIP(hipMemcpyAsync(outs.data(), ro.p, nb * 8, hipMemcpyDeviceToHost, st));
             TS_HIP(hipStreamSynchronize(st));
             std::vector<int32_t> ray_of(nb), longer;
             for (int q = 0; q < nb; q++) {
@@ -467,7 +535,10 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
                 R->path_points[r] = npts_all[r];
                 ray_of[q] = r;
                 if (cnt[q] == -3) longer.push_back(q);
-                else R->counts[r] = cnt[q] >= 0 ? cnt[q] : -1;
+                else {
+                    R->counts[r] = cnt[q] >= 0 ? cnt[q] : -1;
+                    R->outside[r] = cnt[q] >= 0 ? outs[q] : 0.0;
+                }
             }
             pack(rn, rl, cap, rc, cnt, ray_of);
             // rows too long for the small table: formed again with the large one
@@ -476,18 +547,23 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
                 const int64_t capl = kSlotsLarge / 2;
                 dbuf<int32_t> ln((size_t)nl * capl), lc(nl), which(nl);
                 dbuf<float> ll((size_t)nl * capl);
+                dbuf<double> lo(nl);
                 TS_HIP(hipMemcpyAsync(which.p, longer.data(), nl * 4, hipMemcpyHostToDevice, st));
-                RowsArgs B{inv, h, d_path.p, d_jobs.p, d_npts.p, which.p, nl, ln.p, ll.p, capl, lc.p};
+                RowsArgs B{inv, h, d_path.p, d_jobs.p, d_npts.p, which.p, nl, ln.p, ll.p, capl, lc.p,
+                           nbg, d_bg.p, d_bg.p + nbg, d_bg.p + 2 * nbg, lo.p};
                 ray_rows<kSlotsLarge><<<(unsigned)nl, 64, 0, st>>>(B);
                 TS_HIP(hipGetLastError());
                 std::vector<int32_t> cnt2(nl), ray2(nl);
+                std::vector<double> outs2(nl);
                 TS_HIP(hipMemcpyAsync(cnt2.data(), lc.p, nl * 4, hipMemcpyDeviceToHost, st));
+                TS_HIP(hipMemcpyAsync(outs2.data(), lo.p, nl * 8, hipMemcpyDeviceToHost, st));
                 TS_HIP(hipStreamSynchronize(st));
                 for (int i = 0; i < nl; i++) {
                     const int r = jobs[longer[i]].ray;
                     if (cnt2[i] == -3) fail(TS_DECLINED, "ray %d touches more than %lld nodes", r, (long long)capl);
                     ray2[i] = r;
                     R->counts[r] = cnt2[i] >= 0 ? cnt2[i] : -1;
+                    R->outside[r] = cnt2[i] >= 0 ? outs2[i] : 0.0;
                 }
                 pack(ln, ll, capl, lc, cnt2, ray2);
             }
@@ -503,7 +579,7 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
                 const Vec3 a = to_cartesian(events[3 * r], events[3 * r + 1], events[3 * r + 2]);
                 const Vec3 s = to_cartesian(field_station[3 * f], field_station[3 * f + 1], field_station[3 * f + 2]);
                 const int cap = 3 * (int)ceil(vlen(vsub(a, s)) / step_km) + 64;
-                jobs[q] = RayJob{r, f, at, cap};
+                jobs[q] = RayJob{r, f, at, cap, nbg > 0 ? phase_of_ray[r] : 0};
                 at += cap;
             }
             std::vector<int32_t> npts_all(nrays);
@@ -553,7 +629,10 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
         if (nrays > 0) TS_HIP(hipMemcpy(tcg.data(), d_tcg.p, tcg.size() * 8, hipMemcpyDeviceToHost));
         R->tcalc_grad = tcg;
         for (int r = 0; r < nrays; r++)
-            if (R->counts[r] >= 0 && !std::isfinite(tcg[4 * r])) R->counts[r] = -1;  // RayKernels.cs:94
+            if (R->counts[r] >= 0 && !std::isfinite(tcg[4 * r])) {
+                R->counts[r] = -1;  // RayKernels.cs:94
+                R->outside[r] = 0.0;
+            }
         // rows in arrival order (RayKernels.cs:114)
         R->row_start.assign(nrays + 1, 0);
         for (int r = 0; r < nrays; r++) R->row_start[r + 1] = R->row_start[r] + std::max(0, R->counts[r]);
@@ -579,6 +658,21 @@ int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *f
         *out = R.release();
         return TS_OK;
     });
+}
+
+int ts_rays_trace(ts_tableset *tables, const ts_grid *inversion, const double *field_station, int nrays,
+                  const int32_t *field_of_ray, const double *events, double step_km, ts_rays **out, char *reason,
+                  int reason_len)
+{
+    return ts_rays_trace_extended(tables, inversion, field_station, nrays, field_of_ray, nullptr, events, step_km,
+                                  nullptr, out, reason, reason_len);
+}
+
+int ts_rays_outside_time(ts_rays *r, double *outside)
+{
+    if (!r || !outside) return TS_EINVAL;
+    std::copy(r->outside.begin(), r->outside.end(), outside);
+    return TS_OK;
 }
 
 int ts_rays_rows(ts_rays *r, int32_t *counts, int64_t *row_start, int32_t *nodes, float *lengths, double *tcalc_grad,

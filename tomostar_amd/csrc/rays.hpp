@@ -17,6 +17,8 @@ struct ts_rays {
     ts::dbuf<float> lengths;
     std::vector<double> tcalc_grad;   // 4 per arrival: TCalc, Ge, Gn, Gz
     std::vector<int32_t> path_points; // points of each path (diagnostic)
+    std::vector<double> outside;      // per arrival: time of the pieces outside the inversion grid (0: none)
+    bool extended = false;            // traced with a background (pieces outside the grid timed, not attributed)
     double seconds = 0;
 };
 

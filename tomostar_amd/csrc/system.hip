@@ -351,6 +351,7 @@ struct QArgs {
     const double *q;           // q = 1/Q the rows are formed at; null: q0 at every node
     double q0;
     int nrows;
+    const double *outside;     // per arrival: RayRow.OutsideTime; null: no part outside the grid
 };
 
 // dt*/dq of one node: (L s) q, the kernel at q (:337; Predict's term, :310)
@@ -374,7 +375,9 @@ __global__ __launch_bounds__(256) void q_rows(QArgs A, const double *__restrict_
     const int ray = A.ray_of_row[r];
     const int64_t b = A.ray_start[ray], e = A.ray_start[ray + 1];
     const double sg = sigma ? sigma[r] : 1.0;
-    double p = A.q0 * 0.0, t = 0.0, s2 = 0.0;
+    // Predict starts from q0 * OutsideTime, the Q0 estimate's path time from OutsideTime (:231, :309)
+    const double out = A.outside ? A.outside[ray] : 0.0;
+    double p = A.q0 * out, t = out, s2 = 0.0;
     int c = 0;
     for (int64_t k = b; k < e; k++) {
         const int nd = A.nodes[k];
@@ -738,7 +741,10 @@ int ts_q_predict(ts_rays *rays, int nrows, const int32_t *ray_of_row, const doub
         upload(d_ray, ray_of_row, nrows, st);
         upload(d_slow, slowness, n, st);
         if (q) upload(d_q, q, n, st);
-        QArgs A{rays->nodes.p, rays->lengths.p, d_rstart.p, d_ray.p, d_slow.p, q ? d_q.p : nullptr, q0, nrows};
+        dbuf<double> d_out;
+        if (rays->extended && rays->nrays > 0) upload(d_out, rays->outside.data(), rays->outside.size(), st);
+        QArgs A{rays->nodes.p, rays->lengths.p, d_rstart.p, d_ray.p, d_slow.p, q ? d_q.p : nullptr, q0, nrows,
+                rays->extended && rays->nrays > 0 ? d_out.p : nullptr};
         q_rows<<<blocks(nrows, 256), 256, 0, st>>>(A, nullptr, nullptr, nullptr, d_pred.p, d_tt.p, nullptr, nullptr,
                                                    nullptr);
         TS_HIP(hipGetLastError());
@@ -798,7 +804,10 @@ int ts_qsystem_build(ts_context *ctx, const ts_grid *grid, const ts_qsystem_opts
             upload(d_q, q_old, n, st);
             upload(d_m, m_old, n, st);
         }
-        QArgs A{rays->nodes.p, rays->lengths.p, d_rstart.p, d_ray.p, d_slow.p, q_old ? d_q.p : nullptr, o->q0, nrows};
+        dbuf<double> d_out;
+        if (rays->extended && rays->nrays > 0) upload(d_out, rays->outside.data(), rays->outside.size(), st);
+        QArgs A{rays->nodes.p, rays->lengths.p, d_rstart.p, d_ray.p, d_slow.p, q_old ? d_q.p : nullptr, o->q0, nrows,
+                rays->extended && rays->nrays > 0 ? d_out.p : nullptr};
         if (nrows > 0) {
             q_rows<<<blocks(nrows, 256), 256, 0, st>>>(A, d_tstar.p, q_old ? d_m.p : nullptr, d_sigma.p, nullptr,
                                                        nullptr, d_count.p, d_wsum.p, d_rhs.p);

@@ -10,7 +10,9 @@ correction right-hand side with the device product, runs LSQR, and predicts ever
 (RejectOutliers, :412-432), Run (:434-505) and TradeOff (:520-554).
 
 Scope: a plain node grid (no adaptive mesh or parameter lattice), Laplacian smoothing or unweighted
-first differences (no reweighting passes), rays inside the grid (no outside time). The formal
+first differences (no reweighting passes). Rays traced on an extended forward domain carry their
+OutsideTime (the background time of the pieces outside the inversion grid), which enters Predict and the
+Q0 estimate's path time on the device. The formal
 resolution of the last system (:480-497) runs on the device when the settings ask for it.
 """
 from __future__ import annotations
@@ -154,8 +156,8 @@ def _seq_mean(a) -> float:
 
 
 def q_predict(rays: RayBatch, ray_of_row, slowness, q, q0: float):
-    """Predict (:307-312) and the path time sum L s (:232-233) of the given rows on the device; q None:
-    q0 at every node. Returns (pred, tt)."""
+    """Predict (:307-312: q0 * OutsideTime + sum L s q) and the path time OutsideTime + sum L s (:231-233)
+    of the given rows on the device; q None: q0 at every node. Returns (pred, tt)."""
     L = _declare()
     rows = np.ascontiguousarray(ray_of_row, np.int32)
     pred, tt = np.zeros(max(1, rows.size)), np.zeros(max(1, rows.size))

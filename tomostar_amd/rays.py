@@ -28,6 +28,13 @@ class RayRow:
     ge: float
     gn: float
     gz: float
+    outside_time: float = 0.0  # RayRow.OutsideTime: time of the pieces outside the inversion grid
+
+
+class Background(C.Structure):
+    """ts_background: the 1-D model of an extended forward domain (Model/VelocityModel1D.cs)."""
+
+    _fields_ = [("count", C.c_int), ("depth_km", C.c_void_p), ("vp", C.c_void_p), ("vs", C.c_void_p)]
 
 
 def _declare():
@@ -37,6 +44,11 @@ def _declare():
     vp, i, d = C.c_void_p, C.c_int, C.c_double
     L.ts_rays_trace.restype = i
     L.ts_rays_trace.argtypes = [vp, C.POINTER(_lib.TsGrid), vp, i, vp, vp, d, C.POINTER(vp), C.c_char_p, i]
+    L.ts_rays_trace_extended.restype = i
+    L.ts_rays_trace_extended.argtypes = [vp, C.POINTER(_lib.TsGrid), vp, i, vp, vp, vp, d, C.POINTER(Background),
+                                         C.POINTER(vp), C.c_char_p, i]
+    L.ts_rays_outside_time.restype = i
+    L.ts_rays_outside_time.argtypes = [vp, vp]
     L.ts_rays_rows.restype = i
     L.ts_rays_rows.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.ts_rays_stats.restype = i
@@ -53,10 +65,16 @@ def ray_step(forward: GridDefinition, inversion: GridDefinition) -> float:
 
 
 class RayBatch:
-    """Rows of many arrivals traced on the device; arrays in arrival order."""
+    """Rows of many arrivals traced on the device; arrays in arrival order.
+
+    ``background`` = (depth_km, vp, vs) of the 1-D model around an extended forward domain
+    (RayKernels.cs:62-64): the pieces of a ray outside the inversion grid then go to the row's
+    ``outside_time`` (dl / v of the arrival's phase, ``phase_of_ray`` 0 P, 1 S) instead of the grid's
+    edge nodes (RayTracing.Kernel, RayTracing.cs:126-147)."""
 
     def __init__(self, tables: TableSet, inversion: GridDefinition, field_station: np.ndarray,
-                 field_of_ray: np.ndarray, events: np.ndarray, step_km: float | None = None, host_rows: bool = True):
+                 field_of_ray: np.ndarray, events: np.ndarray, step_km: float | None = None, host_rows: bool = True,
+                 background=None, phase_of_ray: np.ndarray | None = None):
         L = _declare()
         fs = np.ascontiguousarray(field_station, np.float64).reshape(-1, 3)
         if fs.shape[0] != tables.nfields:
@@ -69,8 +87,21 @@ class RayBatch:
         g = inversion.to_ts()
         h = C.c_void_p()
         r = reason_buffer()
-        rc = L.ts_rays_trace(tables.handle, C.byref(g), ptr(fs), int(fr.size), ptr(fr), ptr(ev), step, C.byref(h),
-                             r, len(r))
+        if background is None:
+            rc = L.ts_rays_trace(tables.handle, C.byref(g), ptr(fs), int(fr.size), ptr(fr), ptr(ev), step,
+                                 C.byref(h), r, len(r))
+        else:
+            bz, bp, bs = (np.ascontiguousarray(a, np.float64).ravel() for a in background)
+            if not (bz.size == bp.size == bs.size) or bz.size == 0:
+                raise ValueError("the background needs equal, non-empty depth, Vp and Vs arrays")
+            if phase_of_ray is None:
+                raise ValueError("a background needs the phase of every arrival")
+            ph = np.ascontiguousarray(phase_of_ray, np.int32)
+            if ph.size != fr.size:
+                raise ValueError("one phase per arrival")
+            bg = Background(int(bz.size), bz.ctypes.data, bp.ctypes.data, bs.ctypes.data)
+            rc = L.ts_rays_trace_extended(tables.handle, C.byref(g), ptr(fs), int(fr.size), ptr(fr), ptr(ph),
+                                          ptr(ev), step, C.byref(bg), C.byref(h), r, len(r))
         if rc != 0:
             raise _lib.HipError(rc, r.value.decode(errors="replace"))
         self._h = h
@@ -89,6 +120,9 @@ class RayBatch:
         check(L.ts_rays_rows(h, None, None, ptr(self.nodes) if host_rows else None,
                              ptr(self.lengths) if host_rows else None, ptr(self.tcalc_grad), ptr(self.path_points)))
         self.nodes, self.lengths = self.nodes[:nnz], self.lengths[:nnz]
+        self.outside_time = np.zeros(n)
+        if n > 0:
+            check(L.ts_rays_outside_time(h, ptr(self.outside_time)))
         e, s = C.c_int64(), C.c_double()
         check(L.ts_rays_stats(h, C.byref(e), C.byref(s)))
         self.seconds = s.value
@@ -102,7 +136,7 @@ class RayBatch:
             return None
         a, b = self.row_start[r], self.row_start[r + 1]
         t = self.tcalc_grad[r]
-        return RayRow(r, self.nodes[a:b], self.lengths[a:b], *t)
+        return RayRow(r, self.nodes[a:b], self.lengths[a:b], *t, float(self.outside_time[r]))
 
     def rows(self):
         """RayKernels.Compute's output: the rows of the arrivals that have one, in arrival order."""
