@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -262,73 +263,145 @@ __device__ inline int plane_node(const FastArgs<R> &A, int dir, uint32_t pk)
     return plane_node(A, dir, pk, i, j, k);
 }
 
-// The update of plane node pk for the 64 fields of group g (one wavefront, one lane per field). No
-// node of a plane reads another node of the same plane (neighbours lie one or two planes away), so
-// the order of the loads does not change what they read.
-// kLean: the mixed form's arithmetic (godunov_lean) instead of the reference's (FP64: Relax's, bit for
-// bit; FP32: the correctly rounded offset form).
-template <typename R, bool kCount, bool kOff32, int kFlag, bool kLean>
-__device__ inline void update_node(const FastArgs<R> &A, int dir, uint32_t pk, int g, uint64_t act, int lane,
-                                   unsigned counter)
+// The node a lane wakes when the update moves (lanes 0-11): lane 4a + w reads axis a (0 x, 1 y, 2 z)
+// at offset -1, -2, +1, +2 nodes for w = 0, 1, 2, 3. The offset in nodes, fixed per lane for a kernel.
+__device__ inline int wake_offset(int lane, int nx, int nxy)
+{
+    const int ax = (lane >> 2) & 3, w = lane & 3;
+    const int d = w == 0 ? -1 : w == 1 ? -2 : w == 2 ? 1 : 2;
+    return d * (ax == 0 ? 1 : ax == 1 ? nx : nxy);
+}
+
+// What the update of one node reads: its coordinates (wave-uniform), this lane's slowness and time,
+// and the times of its first and second neighbours on each axis (+inf off the grid, and for the second
+// neighbours when no field of the wave is in its second-order phase).
+template <typename R>
+struct NodeLoad {
+    int node, i, j, k;
+    uint8_t fl;
+    uint64_t fixed;
+    R s, cur, mn[3], pl[3], m2[3], p2[3];
+    R wlon0, wlon1, wlat0, wlat1;  // spacing weights of the x and y axes (first, second order)
+};
+
+// The loads of the update of plane node pk for the 64 fields of group g (one wavefront, one lane per
+// field). No node of a plane reads another node of the same plane (neighbours lie one or two planes
+// away), so the loads of several nodes of a plane may be issued before any of them is updated.
+template <typename R, bool kOff32, int kFlag, bool kSecond>
+__device__ __forceinline__ void load_node(const FastArgs<R> &A, int dir, uint32_t pk, int g, int lane, int mdl,
+                                 NodeLoad<R> &L)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
     // every lane of the wave updates the same node (one field per lane): the node's coordinates and
     // addresses are wave-uniform, held in scalar registers
     pk = __builtin_amdgcn_readfirstlane(pk);
-    int i, j, k;
-    const int node = plane_node(A, dir, pk, i, j, k);
+    const int node = plane_node(A, dir, pk, L.i, L.j, L.k);
+    L.node = node;
     const size_t gn = (size_t)g * A.N;
-    uint8_t *flag = A.dirty + gn + node;
-    uint8_t fl = 1;
-    if constexpr (kFlag != kFlagKnown) fl = *flag;
-    if (kFlag == kFlagFirst && fl == 0) return;  // nothing this node reads has moved: the update would return its value
-    const uint64_t fixed = A.seedmask[gn + node];
-    const uint64_t ph = A.phase1[g];
-    const bool any_second = (ph & act) != 0;  // wave-uniform
-    R *Tg = A.T + gn * kLanes;                // this group's times (uniform base)
-    auto T = [&](int q) -> R {                // time of node q for this lane's field
+    L.fl = 1;
+    if constexpr (kFlag != kFlagKnown) L.fl = A.dirty[gn + node];
+    if (kFlag == kFlagFirst && L.fl == 0) return;  // nothing this node reads has moved: the update would return its value
+    L.fixed = A.seedmask[gn + node];
+    const R *Tg = A.T + gn * kLanes;  // this group's times (uniform base)
+    auto T = [&](int q) -> R {        // time of node q for this lane's field
         if constexpr (kOff32) return at_off(Tg, ((uint32_t)q * kLanes + lane) * (uint32_t)sizeof(R));
         else return Tg[(size_t)q * kLanes + lane];
     };
-    const int mdl = A.model[g * kLanes + lane];
-    R s;
-    if constexpr (kOff32) s = at_off(A.slow, ((uint32_t)mdl * (uint32_t)A.N + node) * (uint32_t)sizeof(R));
-    else s = A.slow[(size_t)mdl * A.N + node];
-    const R cur = T(node);
-    const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
-    const R wa[3][2] = {{A.wlon[2 * (k * ny + j)], A.wlon[2 * (k * ny + j) + 1]},
-                        {A.wlat[2 * k], A.wlat[2 * k + 1]}, {A.wr0, A.wr1}};
-    R mn[3], pl[3], m2[3], p2[3];
+    if constexpr (kOff32) L.s = at_off(A.slow, ((uint32_t)mdl * (uint32_t)A.N + node) * (uint32_t)sizeof(R));
+    else L.s = A.slow[(size_t)mdl * A.N + node];
+    L.cur = T(node);
+    // the weights with the node's other loads: a wave waits for its loads in the order they were issued
+    L.wlon0 = A.wlon[2 * (L.k * ny + L.j)];
+    L.wlon1 = A.wlon[2 * (L.k * ny + L.j) + 1];
+    L.wlat0 = A.wlat[2 * L.k];
+    L.wlat1 = A.wlat[2 * L.k + 1];
+    const int pos[3] = {L.i, L.j, L.k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            L.mn[ax] = pos[ax] > 0 ? T(node - st[ax]) : (R)INFINITY;
+            L.pl[ax] = pos[ax] + 1 < n[ax] ? T(node + st[ax]) : (R)INFINITY;
+            L.m2[ax] = kSecond && pos[ax] > 1 ? T(node - 2 * st[ax]) : (R)INFINITY;
+            L.p2[ax] = kSecond && pos[ax] + 2 < n[ax] ? T(node + 2 * st[ax]) : (R)INFINITY;
+        }
+        return;
+    }
+    // FP32: every load is issued, off the grid from the node itself, and replaced by +inf afterwards:
+    // with no load behind a branch the compiler counts the loads in flight exactly, so the update of one
+    // node waits for its own loads only, not for those of the next node already issued.
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
-        mn[ax] = pos[ax] > 0 ? T(node - st[ax]) : (R)INFINITY;
-        pl[ax] = pos[ax] + 1 < n[ax] ? T(node + st[ax]) : (R)INFINITY;
-        m2[ax] = any_second && pos[ax] > 1 ? T(node - 2 * st[ax]) : (R)INFINITY;
-        p2[ax] = any_second && pos[ax] + 2 < n[ax] ? T(node + 2 * st[ax]) : (R)INFINITY;
+        const bool hm = pos[ax] > 0, hp = pos[ax] + 1 < n[ax];
+        const R vm = T(hm ? node - st[ax] : node), vp = T(hp ? node + st[ax] : node);
+        L.mn[ax] = hm ? vm : (R)INFINITY;
+        L.pl[ax] = hp ? vp : (R)INFINITY;
+        if constexpr (kSecond) {
+            const bool hm2 = pos[ax] > 1, hp2 = pos[ax] + 2 < n[ax];
+            const R vm2 = T(hm2 ? node - 2 * st[ax] : node), vp2 = T(hp2 ? node + 2 * st[ax] : node);
+            L.m2[ax] = hm2 ? vm2 : (R)INFINITY;
+            L.p2[ax] = hp2 ? vp2 : (R)INFINITY;
+        } else {
+            L.m2[ax] = L.p2[ax] = (R)INFINITY;
+        }
     }
-    if (fl == 0) return;
-    if (lane == 0) *flag = 0;
-    const bool on = ((act & ~fixed) >> lane) & 1;
-    if (kCount && lane == 0) atomicAdd(&A.evals[counter & 1023], (unsigned long long)__popcll(act & ~fixed));
+}
+
+// The update itself, from what load_node read: the new time, and the node's readers woken when it
+// dropped by more than the wake tolerance. FP32 times: returns the lanes whose field moved the node by
+// more than the settle tolerance (the caller gathers them for the group's changed mask).
+// kLean: the mixed form's arithmetic (godunov_lean) instead of the reference's (FP64: Relax's, bit for
+// bit; FP32: the correctly rounded offset form).
+template <typename R, bool kCount, bool kLean, bool kSecond>
+__device__ __forceinline__ uint64_t finish_node(const FastArgs<R> &A, const NodeLoad<R> &L, int g, uint64_t act,
+                                       uint64_t ph, int lane, unsigned counter, int wake_off)
+{
+    constexpr bool any_second = kSecond;  // some field of the wave in its second-order phase
+    if (L.fl == 0) return 0;
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const int node = L.node, i = L.i, j = L.j, k = L.k;
+    const size_t gn = (size_t)g * A.N;
+    if (lane == 0) A.dirty[gn + node] = 0;
+    const R wa[3][2] = {{L.wlon0, L.wlon1}, {L.wlat0, L.wlat1}, {A.wr0, A.wr1}};
+    const bool on = ((act & ~L.fixed) >> lane) & 1;
+    if (kCount && lane == 0) atomicAdd(&A.evals[counter & 1023], (unsigned long long)__popcll(act & ~L.fixed));
+    const R s = L.s, cur = L.cur;
     bool dropped = false, moved = false;
     if (on) {
         const bool second = (ph >> lane) & 1;
-        R first[3], t2[3];
+        R first[3];
+        bool minus[3];
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
-            // the smaller neighbour of the axis and, second order, the next node beyond it
-            const bool minus = mn[ax] <= pl[ax];
-            first[ax] = minus ? mn[ax] : pl[ax];
-            const R beyond = minus ? m2[ax] : p2[ax];
-            t2[ax] = (second && isfinite(first[ax])) ? beyond : (R)INFINITY;
+            // the smaller neighbour of the axis
+            minus[ax] = L.mn[ax] <= L.pl[ax];
+            first[ax] = minus[ax] ? L.mn[ax] : L.pl[ax];
         }
-        R next;
-        if constexpr (kLean && sizeof(R) == 4) next = (s > 0.0f) ? godunov_lean32(s, first, t2, wa) : (R)INFINITY;
-        else if constexpr (kLean) next = (s > 0.0) ? godunov_lean64(s, first, t2, wa) : (R)INFINITY;
-        else if constexpr (sizeof(R) == 4) next = (s > 0.0f) ? godunov_offset(s, first, t2, wa) : (R)INFINITY;
-        else next = (s > 0.0) ? godunov_abs_w<double>(s, first, t2, wa) : (R)INFINITY;
+        // second order: the next node beyond the smaller neighbour
+        auto beyond = [&](R t2[3]) {
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++)
+                t2[ax] = (second && isfinite(first[ax])) ? (minus[ax] ? L.m2[ax] : L.p2[ax]) : (R)INFINITY;
+        };
+        R next, t2[3];
+        if constexpr (kLean && sizeof(R) == 4) {
+            beyond(t2);
+            next = (s > 0.0f) ? godunov_lean32(s, first, t2, wa) : (R)INFINITY;
+        } else if constexpr (kLean) {
+            beyond(t2);
+            next = (s > 0.0) ? godunov_lean64(s, first, t2, wa) : (R)INFINITY;
+        } else if constexpr (sizeof(R) == 4) {
+            // no field of the wave in its second-order phase (wave-uniform): the first-order arithmetic
+            if (!any_second) next = (s > 0.0f) ? godunov_offset_first(s, first, wa) : (R)INFINITY;
+            else {
+                beyond(t2);
+                next = (s > 0.0f) ? godunov_offset(s, first, t2, wa) : (R)INFINITY;
+            }
+        } else {
+            beyond(t2);
+            next = (s > 0.0) ? godunov_abs_w<double>(s, first, t2, wa) : (R)INFINITY;
+        }
         if (next < cur) {
-            Tg[(size_t)node * kLanes + lane] = next;
+            A.T[(gn + node) * kLanes + lane] = next;
             dropped = !isfinite(cur) || (double)(cur - next) > A.wake_tol;
             if constexpr (sizeof(R) == 4)
                 moved = !isfinite(cur) || (cur - next) > fmaxf((float)kSettleTol, A.settle_ulps * 1.1920929e-7f * cur);
@@ -338,24 +411,67 @@ __device__ inline void update_node(const FastArgs<R> &A, int dir, uint32_t pk, i
     }
     const uint64_t woke = __ballot(dropped);
     if (woke) {
-        // Wake the nodes that read this one: one and two steps on each axis (lanes 0-11) when a field of
-        // the group is in its second-order phase, else one step (lanes 0-5) - a first-order update reads
-        // no node two steps away, and a field entering its second-order phase wakes every node.
-        if (lane < (any_second ? 12 : 6)) {
-            const int ax = any_second ? lane >> 2 : lane >> 1;
-            const int pos = ax == 0 ? i : ax == 1 ? j : k, n = ax == 0 ? nx : ax == 1 ? ny : nz;
-            const int st = ax == 0 ? 1 : ax == 1 ? nx : nxy;
-            const int which = any_second ? lane & 3 : (lane & 1) * 2;
-            const int d = which == 0 ? -1 : which == 1 ? -2 : which == 2 ? 1 : 2;
-            const int np = pos + d;
-            if (np >= 0 && np < n) A.dirty[gn + node + d * st] = 1;
-        }
+        // Wake the nodes that read this one: one and two steps on each axis (lanes 0-11, wake_offset)
+        // when a field of the group is in its second-order phase, else one step - a first-order update
+        // reads no node two steps away, and a field entering its second-order phase wakes every node.
+        // Bit 4a + w of the wave-uniform mask: the lane's node lies in the grid.
+        const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz};
+        uint32_t in = 0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++)
+            in |= (uint32_t)((pos[ax] >= 1) | (pos[ax] >= 2) << 1 | (pos[ax] + 1 < n[ax]) << 2 |
+                             (pos[ax] + 2 < n[ax]) << 3) << (4 * ax);
+        if (!any_second) in &= 0x555u;
+        if (((uint64_t)in >> lane) & 1) A.dirty[gn + node + wake_off] = 1;
     }
     const uint64_t mv = __ballot(moved);
-    if (mv && lane == 0) {
-        uint64_t seen = __hip_atomic_load(&A.changed[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (mv & ~seen) atomicOr((unsigned long long *)&A.changed[g], (unsigned long long)mv);
+    if constexpr (sizeof(R) == 8) {
+        // FP64 times: into the group's changed mask at once, when the mask (read at the device's scope)
+        // lacks them - measured faster here than gathering them per wave
+        if (mv && lane == 0) {
+            const uint64_t seen = __hip_atomic_load(&A.changed[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (mv & ~seen) atomicOr((unsigned long long *)&A.changed[g], (unsigned long long)mv);
+        }
+        return 0;
     }
+    return mv;
+}
+
+// The lanes a wave saw move, into the group's changed mask: at most one atomic per wave, at its end, and
+// none when the mask as this XCD's cache holds it already has them (the mask only gains bits during a
+// round, so a stale copy can only ask for a redundant atomic, never skip a needed one).
+__device__ inline void mark_changed(uint64_t *changed, uint64_t mv, int lane)
+{
+    if (lane == 0 && mv) {
+        const uint64_t seen = __hip_atomic_load(changed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (mv & ~seen) atomicOr((unsigned long long *)changed, (unsigned long long)mv);
+    }
+}
+
+// Kernels that only ever see one kind of wave compile that body alone (one body runs faster than two
+// in one kernel): the mixed form's FP64 kernels run its second-order phase only (and the second-order
+// body is right for any wave, only slower for a first-order one); rounds on FP32 times hold every field
+// at the end of its first-order phase (hold in launch_round), so FP32 kernels see first-order waves only.
+template <typename R, bool kLean>
+constexpr bool kSecondOnly = kLean && sizeof(R) == 8;
+template <typename R>
+constexpr bool kFirstOnly = sizeof(R) == 4;
+
+// The nodes a wave of eik_fast_plane updates: plane nodes a, a + stride, ... (kSecond: some field of
+// the group in its second-order phase, so the loads two nodes away are issued).
+template <typename R, bool kCount, bool kOff32, int kFlag, bool kLean, bool kSecond>
+__device__ __forceinline__ void plane_nodes(const FastArgs<R> &A, int dir, int off, int cnt, int g, uint64_t act,
+                                            uint64_t ph, int lane, int a, int stride)
+{
+    const int wake_off = wake_offset(lane, A.nx, A.nx * A.ny);
+    const int mdl = A.model[g * kLanes + lane];
+    uint64_t mv = 0;
+    for (; a < cnt; a += stride) {
+        NodeLoad<R> L;
+        load_node<R, kOff32, kFlag, kSecond>(A, dir, A.planes[off + a], g, lane, mdl, L);
+        mv |= finish_node<R, kCount, kLean, kSecond>(A, L, g, act, ph, lane, blockIdx.x, wake_off);
+    }
+    mark_changed(&A.changed[g], mv, lane);
 }
 
 // One diagonal plane of one sweep ordering for the groups g0 .. g0 + gridDim.y - 1: each wavefront
@@ -375,14 +491,54 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int di
     const int per_xcd = (gridDim.x + 7) / 8;
     const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     const int stride = 8 * per_xcd * WPB;
-    for (int a = lb * WPB + (threadIdx.x >> 6); a < cnt; a += stride)
-        update_node<R, kCount, kOff32, kFlag, kLean>(A, dir, A.planes[off + a], g, act, lane, blockIdx.x);
+    const int a = lb * WPB + (threadIdx.x >> 6);
+    const uint64_t ph = A.phase1[g];
+    // The mixed form's FP64 kernels (kLean) run its second-order phase only: one body, the kernel's code
+    // half the size (the second-order body is right for any wave, only slower for a first-order one).
+    if constexpr (kFirstOnly<R>) plane_nodes<R, kCount, kOff32, kFlag, kLean, false>(A, dir, off, cnt, g, act, ph, lane, a, stride);
+    else if (kSecondOnly<R, kLean> || (ph & act) != 0)
+        plane_nodes<R, kCount, kOff32, kFlag, kLean, true>(A, dir, off, cnt, g, act, ph, lane, a, stride);
+    else plane_nodes<R, kCount, kOff32, kFlag, kLean, false>(A, dir, off, cnt, g, act, ph, lane, a, stride);
+}
+
+// The dirty nodes of a wave of eik_fast_plane_sparse (todo: the lanes holding them in pk), in plane order.
+template <typename R, bool kCount, bool kOff32, bool kLean, bool kPair, bool kSecond>
+__device__ __forceinline__ void sparse_nodes(const FastArgs<R> &A, int dir, int g, uint64_t act, uint64_t ph,
+                                             int lane, uint32_t pk, uint64_t todo)
+{
+    const int wake_off = wake_offset(lane, A.nx, A.nx * A.ny);
+    const int mdl = A.model[g * kLanes + lane];
+    uint64_t mv = 0;
+    if constexpr (kPair && !kSecond) {
+        // two dirty nodes at a time: the second node's loads are in flight while the first is updated
+        // (first-order waves only: the second-order loads of two nodes would cost the kernel its occupancy)
+        while (todo) {
+            const uint32_t p0 = __shfl(pk, __builtin_ctzll(todo));
+            todo &= todo - 1;
+            const bool two = todo != 0;
+            const uint32_t p1 = __shfl(pk, two ? __builtin_ctzll(todo) : 0);
+            if (two) todo &= todo - 1;
+            NodeLoad<R> L0, L1;
+            load_node<R, kOff32, kFlagKnown, kSecond>(A, dir, p0, g, lane, mdl, L0);
+            // a lone node loads itself twice rather than branch around the second loads
+            load_node<R, kOff32, kFlagKnown, kSecond>(A, dir, two ? p1 : p0, g, lane, mdl, L1);
+            mv |= finish_node<R, kCount, kLean, kSecond>(A, L0, g, act, ph, lane, blockIdx.x, wake_off);
+            if (two) mv |= finish_node<R, kCount, kLean, kSecond>(A, L1, g, act, ph, lane, blockIdx.x, wake_off);
+        }
+    } else {
+        for (; todo; todo &= todo - 1) {
+            NodeLoad<R> L;
+            load_node<R, kOff32, kFlagKnown, kSecond>(A, dir, __shfl(pk, __builtin_ctzll(todo)), g, lane, mdl, L);
+            mv |= finish_node<R, kCount, kLean, kSecond>(A, L, g, act, ph, lane, blockIdx.x, wake_off);
+        }
+    }
+    mark_changed(&A.changed[g], mv, lane);
 }
 
 // Rounds in which most nodes are clean: each wavefront reads the dirty flags of M consecutive plane
 // nodes at once (lanes 0 .. M-1) and then updates its dirty ones in plane order, so a clean stretch of
 // the plane costs one load per M nodes and M times fewer waves are launched.
-template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean>
+template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean, bool kPair = (sizeof(R) == 4)>
 __global__ __launch_bounds__(64 * WPB) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
@@ -399,10 +555,12 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane_sparse(FastArgs<R> A,
         pk = A.planes[off + a0 + lane];
         dirty = A.dirty[(size_t)g * A.N + plane_node(A, dir, pk)] != 0;
     }
-    for (uint64_t todo = __ballot(dirty); todo; todo &= todo - 1) {
-        const uint32_t p = __shfl(pk, __builtin_ctzll(todo));
-        update_node<R, kCount, kOff32, kFlagKnown, kLean>(A, dir, p, g, act, lane, blockIdx.x);
-    }
+    const uint64_t ph = A.phase1[g];
+    const uint64_t todo = __ballot(dirty);
+    if constexpr (kFirstOnly<R>) sparse_nodes<R, kCount, kOff32, kLean, kPair, false>(A, dir, g, act, ph, lane, pk, todo);
+    else if (kSecondOnly<R, kLean> || (ph & act) != 0)
+        sparse_nodes<R, kCount, kOff32, kLean, kPair, true>(A, dir, g, act, ph, lane, pk, todo);
+    else sparse_nodes<R, kCount, kOff32, kLean, kPair, false>(A, dir, g, act, ph, lane, pk, todo);
 }
 
 // Per-field bookkeeping after a round (one wave per group): a field whose round moved no node by

@@ -76,7 +76,7 @@ __host__ __device__ inline void spacing_weights(R h, R w[2])
 // godunov_abs with the spacing weights of the three axes given (wa[a][0] first order, wa[a][1]
 // second order): the same arithmetic in the same order, without the six divisions that form them.
 template <typename R>
-__device__ inline R godunov_abs_w(R s, const R first[3], const R t2[3], const R wa[3][2])
+__device__ __forceinline__ R godunov_abs_w(R s, const R first[3], const R t2[3], const R wa[3][2])
 {
     const R inf = (R)INFINITY;
     R te[3], we[3], tn[3];
@@ -127,7 +127,7 @@ __device__ inline R godunov_abs_w(R s, const R first[3], const R t2[3], const R 
 // FP32 update in the offset form (the float path of EikonalOpenCl.cs:454-496, :568-606 restated for
 // plain float times): every term is taken relative to r, the smallest upwind neighbour, so the
 // quadratic's terms are cell crossing times and only the final r + x rounds at the size of t.
-__device__ inline float godunov_offset(float s, const float first[3], const float t2[3], const float wa[3][2])
+__device__ __forceinline__ float godunov_offset(float s, const float first[3], const float t2[3], const float wa[3][2])
 {
     const float inf = INFINITY;
     float r = inf;
@@ -181,6 +181,61 @@ __device__ inline float godunov_offset(float s, const float first[3], const floa
     return found ? r + cand : inf;
 }
 
+// godunov_offset on a node none of whose axes has a second-order term (every t2 +inf, as in a
+// first-order phase): the same operations in the same order on what is left of them, so the same bits.
+// Without second-order terms te[a] = tn[a] = first[a] - r (+inf for an absent axis, as r is finite) and
+// we[a] = wa[a][0], so the divisions by 3, the tn payload of the ordering network and its selections drop
+// out.
+__device__ __forceinline__ float godunov_offset_first(float s, const float first[3], const float wa[3][2])
+{
+    const float inf = INFINITY;
+    float r = inf;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (first[a] < r) r = first[a];
+    if (!(r < inf)) return inf;
+    float te[3], we[3];
+    int used = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        te[a] = first[a] - r;
+        we[a] = wa[a][0];
+        used += __builtin_isfinite(first[a]);
+    }
+#define TS_SWAP(x, y) { float t_ = x; x = y; y = t_; }
+    if (te[0] > te[1]) { TS_SWAP(te[0], te[1]) TS_SWAP(we[0], we[1]) }
+    if (te[1] > te[2]) {
+        TS_SWAP(te[1], te[2]) TS_SWAP(we[1], we[2])
+        if (te[0] > te[1]) { TS_SWAP(te[0], te[1]) TS_SWAP(we[0], we[1]) }
+    }
+#undef TS_SWAP
+    const float base = te[0];
+    float cand = inf;
+    bool found = false;
+#pragma unroll
+    for (int terms = 3; terms >= 1; terms--) {
+        if (found || terms > used) continue;
+        float sw = 0.0f, swd = 0.0f, swd2 = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if (q >= terms) continue;
+            float d = te[q] - base;
+            sw += we[q];
+            swd += we[q] * d;
+            swd2 += we[q] * d * d;
+        }
+        float disc = swd * swd - sw * (swd2 - s * s);
+        if (disc < 0.0f) continue;
+        float root = base + (swd + sqrtf(disc)) / sw;
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            if (q < terms && root < te[q]) ok = false;
+        if (ok) { cand = root; found = true; }
+    }
+    return found ? r + cand : inf;
+}
+
 // The mixed form's updates (no bit-for-bit obligation: the mixed form is held to 1e-4 of Relax). The
 // terms of the reference's quadratic (the same selection as godunov_abs: the largest number of terms
 // whose root is causal, SphericalEikonal.cs:365-395) are formed as offsets from r, the smallest upwind
@@ -199,7 +254,7 @@ __device__ inline float godunov_offset(float s, const float first[3], const floa
 // differ by an ulp here and there, it settles elsewhere on large regions (400 fields of 201 x 201 x 101:
 // largest distance to Relax 5.7e-5 against 4.6e-6, 9e4 nodes of one field moved by more than 1e-6 s
 // against 208), so the first-order phase keeps the correctly rounded offset form.
-__device__ inline float godunov_lean32(float s, const float first[3], const float t2[3], const float wa[3][2])
+__device__ __forceinline__ float godunov_lean32(float s, const float first[3], const float t2[3], const float wa[3][2])
 {
     const float r = fminf(first[0], fminf(first[1], first[2]));
     const float inf = INFINITY;
@@ -238,22 +293,22 @@ __device__ inline float godunov_lean32(float s, const float first[3], const floa
 }
 
 // 1 / x and 1 / sqrt(x) for x > 0: the hardware's estimates and two Newton steps
-__device__ inline double rcp_nr(double x)
+__device__ __forceinline__ double rcp_nr(double x)
 {
     double y = __builtin_amdgcn_rcp(x);
     y = fma(y, fma(-x, y, 1.0), y);
     return fma(y, fma(-x, y, 1.0), y);
 }
-__device__ inline double rsq_nr(double x)
+__device__ __forceinline__ double rsq_nr(double x)
 {
     double y = __builtin_amdgcn_rsq(x);
     y = y * fma(-0.5 * x * y, y, 1.5);
     return y * fma(-0.5 * x * y, y, 1.5);
 }
 // sqrt(x) for x >= 0 (0 for x <= 0)
-__device__ inline double sqrt_nr(double x) { return x > 0 ? x * rsq_nr(x) : 0.0; }
+__device__ __forceinline__ double sqrt_nr(double x) { return x > 0 ? x * rsq_nr(x) : 0.0; }
 
-__device__ inline double godunov_lean64(double s, const double first[3], const double t2[3], const double wa[3][2])
+__device__ __forceinline__ double godunov_lean64(double s, const double first[3], const double t2[3], const double wa[3][2])
 {
     const double r = fmin(first[0], fmin(first[1], first[2]));
     const double inf = INFINITY;
