@@ -287,7 +287,7 @@ struct NodeLoad {
 // The loads of the update of plane node pk for the 64 fields of group g (one wavefront, one lane per
 // field). No node of a plane reads another node of the same plane (neighbours lie one or two planes
 // away), so the loads of several nodes of a plane may be issued before any of them is updated.
-template <typename R, bool kOff32, int kFlag, bool kSecond>
+template <typename R, bool kOff32, int kFlag, bool kSecond, bool kBranchLoads = (sizeof(R) == 8)>
 __device__ __forceinline__ void load_node(const FastArgs<R> &A, int dir, uint32_t pk, int g, int lane, int mdl,
                                  NodeLoad<R> &L)
 {
@@ -316,7 +316,8 @@ __device__ __forceinline__ void load_node(const FastArgs<R> &A, int dir, uint32_
     L.wlat0 = A.wlat[2 * L.k];
     L.wlat1 = A.wlat[2 * L.k + 1];
     const int pos[3] = {L.i, L.j, L.k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
-    if constexpr (sizeof(R) == 8) {
+    if constexpr (kBranchLoads) {
+        // one node per wave at a time (FP64): loads off the grid are skipped
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
             L.mn[ax] = pos[ax] > 0 ? T(node - st[ax]) : (R)INFINITY;
@@ -326,9 +327,10 @@ __device__ __forceinline__ void load_node(const FastArgs<R> &A, int dir, uint32_
         }
         return;
     }
-    // FP32: every load is issued, off the grid from the node itself, and replaced by +inf afterwards:
-    // with no load behind a branch the compiler counts the loads in flight exactly, so the update of one
-    // node waits for its own loads only, not for those of the next node already issued.
+    // Two nodes per wave at a time (FP32): every load is issued, off the grid from the node itself, and
+    // replaced by +inf afterwards: with no load behind a branch the compiler counts the loads in flight
+    // exactly, so the update of one node waits for its own loads only, not for those of the next node
+    // already issued.
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
         const bool hm = pos[ax] > 0, hp = pos[ax] + 1 < n[ax];
@@ -468,7 +470,7 @@ __device__ __forceinline__ void plane_nodes(const FastArgs<R> &A, int dir, int o
     uint64_t mv = 0;
     for (; a < cnt; a += stride) {
         NodeLoad<R> L;
-        load_node<R, kOff32, kFlag, kSecond>(A, dir, A.planes[off + a], g, lane, mdl, L);
+        load_node<R, kOff32, kFlag, kSecond, false>(A, dir, A.planes[off + a], g, lane, mdl, L);
         mv |= finish_node<R, kCount, kLean, kSecond>(A, L, g, act, ph, lane, blockIdx.x, wake_off);
     }
     mark_changed(&A.changed[g], mv, lane);
@@ -501,36 +503,33 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int di
     else plane_nodes<R, kCount, kOff32, kFlag, kLean, false>(A, dir, off, cnt, g, act, ph, lane, a, stride);
 }
 
-// The dirty nodes of a wave of eik_fast_plane_sparse (todo: the lanes holding them in pk), in plane order.
-template <typename R, bool kCount, bool kOff32, bool kLean, bool kPair, bool kSecond>
+// The dirty nodes of a wave of eik_fast_plane_sparse (todo: the lanes holding them in pk), in plane order,
+// K at a time: the loads of K nodes are issued before the first of them is updated, so the later nodes'
+// loads are in flight while the earlier ones are updated.
+template <typename R, bool kCount, bool kOff32, bool kLean, int K, bool kSecond>
 __device__ __forceinline__ void sparse_nodes(const FastArgs<R> &A, int dir, int g, uint64_t act, uint64_t ph,
                                              int lane, uint32_t pk, uint64_t todo)
 {
     const int wake_off = wake_offset(lane, A.nx, A.nx * A.ny);
     const int mdl = A.model[g * kLanes + lane];
     uint64_t mv = 0;
-    if constexpr (kPair && !kSecond) {
-        // two dirty nodes at a time: the second node's loads are in flight while the first is updated
-        // (first-order waves only: the second-order loads of two nodes would cost the kernel its occupancy)
-        while (todo) {
-            const uint32_t p0 = __shfl(pk, __builtin_ctzll(todo));
-            todo &= todo - 1;
-            const bool two = todo != 0;
-            const uint32_t p1 = __shfl(pk, two ? __builtin_ctzll(todo) : 0);
-            if (two) todo &= todo - 1;
-            NodeLoad<R> L0, L1;
-            load_node<R, kOff32, kFlagKnown, kSecond>(A, dir, p0, g, lane, mdl, L0);
-            // a lone node loads itself twice rather than branch around the second loads
-            load_node<R, kOff32, kFlagKnown, kSecond>(A, dir, two ? p1 : p0, g, lane, mdl, L1);
-            mv |= finish_node<R, kCount, kLean, kSecond>(A, L0, g, act, ph, lane, blockIdx.x, wake_off);
-            if (two) mv |= finish_node<R, kCount, kLean, kSecond>(A, L1, g, act, ph, lane, blockIdx.x, wake_off);
+    while (todo) {
+        uint32_t p[K];
+        bool valid[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            valid[q] = todo != 0;
+            p[q] = __shfl(pk, valid[q] ? __builtin_ctzll(todo) : 0);
+            // a missing node loads the first one again rather than branch around its loads
+            if (!valid[q]) p[q] = p[0];
+            if (valid[q]) todo &= todo - 1;
         }
-    } else {
-        for (; todo; todo &= todo - 1) {
-            NodeLoad<R> L;
-            load_node<R, kOff32, kFlagKnown, kSecond>(A, dir, __shfl(pk, __builtin_ctzll(todo)), g, lane, mdl, L);
-            mv |= finish_node<R, kCount, kLean, kSecond>(A, L, g, act, ph, lane, blockIdx.x, wake_off);
-        }
+        NodeLoad<R> L[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) load_node<R, kOff32, kFlagKnown, kSecond, (K == 1)>(A, dir, p[q], g, lane, mdl, L[q]);
+#pragma unroll
+        for (int q = 0; q < K; q++)
+            if (valid[q]) mv |= finish_node<R, kCount, kLean, kSecond>(A, L[q], g, act, ph, lane, blockIdx.x, wake_off);
     }
     mark_changed(&A.changed[g], mv, lane);
 }
@@ -538,7 +537,7 @@ __device__ __forceinline__ void sparse_nodes(const FastArgs<R> &A, int dir, int 
 // Rounds in which most nodes are clean: each wavefront reads the dirty flags of M consecutive plane
 // nodes at once (lanes 0 .. M-1) and then updates its dirty ones in plane order, so a clean stretch of
 // the plane costs one load per M nodes and M times fewer waves are launched.
-template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean, bool kPair = (sizeof(R) == 4)>
+template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean, int K>
 __global__ __launch_bounds__(64 * WPB) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
@@ -557,10 +556,10 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane_sparse(FastArgs<R> A,
     }
     const uint64_t ph = A.phase1[g];
     const uint64_t todo = __ballot(dirty);
-    if constexpr (kFirstOnly<R>) sparse_nodes<R, kCount, kOff32, kLean, kPair, false>(A, dir, g, act, ph, lane, pk, todo);
+    if constexpr (kFirstOnly<R>) sparse_nodes<R, kCount, kOff32, kLean, K, false>(A, dir, g, act, ph, lane, pk, todo);
     else if (kSecondOnly<R, kLean> || (ph & act) != 0)
-        sparse_nodes<R, kCount, kOff32, kLean, kPair, true>(A, dir, g, act, ph, lane, pk, todo);
-    else sparse_nodes<R, kCount, kOff32, kLean, kPair, false>(A, dir, g, act, ph, lane, pk, todo);
+        sparse_nodes<R, kCount, kOff32, kLean, K, true>(A, dir, g, act, ph, lane, pk, todo);
+    else sparse_nodes<R, kCount, kOff32, kLean, K, false>(A, dir, g, act, ph, lane, pk, todo);
 }
 
 // Per-field bookkeeping after a round (one wave per group): a field whose round moved no node by
@@ -937,10 +936,19 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
 
 constexpr int kWPB = 4;
 constexpr int kSparse = 2;  // round order after kFlagWithLoads, kFlagFirst: kSparseNodes flags per wave, then the dirty nodes
-#ifndef TS_SPARSE_NODES
-#define TS_SPARSE_NODES 2
+#ifndef TS_SPARSE_NODES32
+#define TS_SPARSE_NODES32 4
 #endif
-constexpr int kSparseNodes = TS_SPARSE_NODES;
+#ifndef TS_INFLIGHT32
+#define TS_INFLIGHT32 4
+#endif
+// Flags read per wave of a sparse launch (M) and dirty nodes whose loads are in flight together (K): FP32
+// waves carry the loads of K first-order nodes within the registers of full occupancy, FP64 waves (twelve
+// FP64 neighbours) one.
+template <typename R>
+constexpr int kSparseNodes = sizeof(R) == 4 ? TS_SPARSE_NODES32 : 2;
+template <typename R>
+constexpr int kInFlight = sizeof(R) == 4 ? TS_INFLIGHT32 : 1;
 
 // Cap on the blocks of a plane launch over all its groups (TOMOSTAR_PLANE_BLOCKS): waves stride over
 // the plane beyond it. Measured on the 400-field case: fewer, longer-lived waves are slower (the
@@ -974,12 +982,13 @@ void launch_planes(ts_tableset &t, R *T, hipStream_t st, int g0, int gc)
         for (int p = 0; p < P; p++) {
             int off = t.plane_off[p], cnt = t.plane_off[p + 1] - off;
             if constexpr (kOrder == kSparse) {
-                const unsigned nb = blocks_for(cnt, kWPB * kSparseNodes);
+                constexpr int M = kSparseNodes<R>, K = kInFlight<R>;
+                const unsigned nb = blocks_for(cnt, kWPB * M);
                 dim3 grid((nb + 7) / 8 * 8, gc);
                 if (off32)
-                    eik_fast_plane_sparse<R, kWPB, kSparseNodes, kCount, true, kLean><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                    eik_fast_plane_sparse<R, kWPB, M, kCount, true, kLean, K><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
                 else
-                    eik_fast_plane_sparse<R, kWPB, kSparseNodes, kCount, false, kLean><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                    eik_fast_plane_sparse<R, kWPB, M, kCount, false, kLean, K><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
             } else {
                 // a multiple of 8 blocks: each XCD gets the same number of slices
                 const unsigned nb = std::min<unsigned>(blocks_for(cnt, kWPB), std::max(1, plane_blocks(t) / gc));
@@ -1084,7 +1093,11 @@ void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count,
 //   with every node of its group dirty again.
 // Sparse rounds first took the solve from 1.26 s to 1.20 s; with the rounds above, 1, 2, 4 and 8 flags
 // per wave (kSparseNodes) take it 0.986, 0.946, 0.959 and 1.024 s (16 or 32: slower still; fewer waves
-// are left to hide the latency of the dirty nodes).
+// are left to hide the latency of the dirty nodes). With the loads of several dirty nodes in flight
+// (kInFlight), the FP32 phase takes 0.237 s at 2 flags and 2 nodes, 0.227 s at 4 and 4, 0.239 s at 8 and
+// 4; 64 flags per wave from a late round on is slower in every round tried (FP64 rounds 2, 3, 4: 1.103,
+// 0.761, 0.584 s against 0.569 s; FP32 rounds 1, 2: 0.617, 0.407 s against 0.226 s) - the late rounds
+// still hold dirty nodes enough that a wave working through 64 flags' worth becomes the launch's tail.
 int round_order(bool all_dirty, bool one_phase, int round)
 {
     int first = 0;
