@@ -292,7 +292,8 @@ __device__ __forceinline__ float godunov_lean32(float s, const float first[3], c
     return cand < inf ? r + cand : inf;
 }
 
-// 1 / x and 1 / sqrt(x) for x > 0: the hardware's estimates and two Newton steps
+// 1 / x and 1 / sqrt(x) for x > 0: the hardware's estimates and two Newton steps (one step: the
+// 400-field solve 0.5 % faster, which does not pay for the weaker bound)
 __device__ __forceinline__ double rcp_nr(double x)
 {
     double y = __builtin_amdgcn_rcp(x);
@@ -310,7 +311,9 @@ __device__ __forceinline__ double sqrt_nr(double x) { return x > 0 ? x * rsq_nr(
 
 __device__ __forceinline__ double godunov_lean64(double s, const double first[3], const double t2[3], const double wa[3][2])
 {
-    const double r = fmin(first[0], fmin(first[1], first[2]));
+    // the smallest neighbour (compare and select: the times are never NaN, so no quieting is needed)
+    const double r01 = first[1] < first[0] ? first[1] : first[0];
+    const double r = first[2] < r01 ? first[2] : r01;
     const double inf = INFINITY;
     if (!(r < inf)) return inf;
     double te[3], we[3], tn[3];
