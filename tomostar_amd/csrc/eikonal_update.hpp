@@ -76,7 +76,7 @@ __host__ __device__ inline void spacing_weights(R h, R w[2])
 // godunov_abs with the spacing weights of the three axes given (wa[a][0] first order, wa[a][1]
 // second order): the same arithmetic in the same order, without the six divisions that form them.
 template <typename R>
-__device__ __forceinline__ R godunov_abs_w(R s, const R first[3], const R t2[3], const R wa[3][2])
+__device__ inline R godunov_abs_w(R s, const R first[3], const R t2[3], const R wa[3][2])
 {
     const R inf = (R)INFINITY;
     R te[3], we[3], tn[3];
@@ -127,7 +127,7 @@ __device__ __forceinline__ R godunov_abs_w(R s, const R first[3], const R t2[3],
 // FP32 update in the offset form (the float path of EikonalOpenCl.cs:454-496, :568-606 restated for
 // plain float times): every term is taken relative to r, the smallest upwind neighbour, so the
 // quadratic's terms are cell crossing times and only the final r + x rounds at the size of t.
-__device__ __forceinline__ float godunov_offset(float s, const float first[3], const float t2[3], const float wa[3][2])
+__device__ inline float godunov_offset(float s, const float first[3], const float t2[3], const float wa[3][2])
 {
     const float inf = INFINITY;
     float r = inf;
@@ -186,7 +186,7 @@ __device__ __forceinline__ float godunov_offset(float s, const float first[3], c
 // Without second-order terms te[a] = tn[a] = first[a] - r (+inf for an absent axis, as r is finite) and
 // we[a] = wa[a][0], so the divisions by 3, the tn payload of the ordering network and its selections drop
 // out.
-__device__ __forceinline__ float godunov_offset_first(float s, const float first[3], const float wa[3][2])
+__device__ inline float godunov_offset_first(float s, const float first[3], const float wa[3][2])
 {
     const float inf = INFINITY;
     float r = inf;
@@ -254,7 +254,7 @@ __device__ __forceinline__ float godunov_offset_first(float s, const float first
 // differ by an ulp here and there, it settles elsewhere on large regions (400 fields of 201 x 201 x 101:
 // largest distance to Relax 5.7e-5 against 4.6e-6, 9e4 nodes of one field moved by more than 1e-6 s
 // against 208), so the first-order phase keeps the correctly rounded offset form.
-__device__ __forceinline__ float godunov_lean32(float s, const float first[3], const float t2[3], const float wa[3][2])
+__device__ inline float godunov_lean32(float s, const float first[3], const float t2[3], const float wa[3][2])
 {
     const float r = fminf(first[0], fminf(first[1], first[2]));
     const float inf = INFINITY;
@@ -294,22 +294,22 @@ __device__ __forceinline__ float godunov_lean32(float s, const float first[3], c
 
 // 1 / x and 1 / sqrt(x) for x > 0: the hardware's estimates and two Newton steps (one step: the
 // 400-field solve 0.5 % faster, which does not pay for the weaker bound)
-__device__ __forceinline__ double rcp_nr(double x)
+__device__ inline double rcp_nr(double x)
 {
     double y = __builtin_amdgcn_rcp(x);
     y = fma(y, fma(-x, y, 1.0), y);
     return fma(y, fma(-x, y, 1.0), y);
 }
-__device__ __forceinline__ double rsq_nr(double x)
+__device__ inline double rsq_nr(double x)
 {
     double y = __builtin_amdgcn_rsq(x);
     y = y * fma(-0.5 * x * y, y, 1.5);
     return y * fma(-0.5 * x * y, y, 1.5);
 }
 // sqrt(x) for x >= 0 (0 for x <= 0)
-__device__ __forceinline__ double sqrt_nr(double x) { return x > 0 ? x * rsq_nr(x) : 0.0; }
+__device__ inline double sqrt_nr(double x) { return x > 0 ? x * rsq_nr(x) : 0.0; }
 
-__device__ __forceinline__ double godunov_lean64(double s, const double first[3], const double t2[3], const double wa[3][2])
+__device__ inline double godunov_lean64(double s, const double first[3], const double t2[3], const double wa[3][2])
 {
     // the smallest neighbour (compare and select: the times are never NaN, so no quieting is needed)
     const double r01 = first[1] < first[0] ? first[1] : first[0];
