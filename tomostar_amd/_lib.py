@@ -67,6 +67,7 @@ def load() -> C.CDLL:
     _sig(L, "ts_tables_rounds", _i, _vp, _vp)
     _sig(L, "ts_tables_stats", _i, _vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d), C.POINTER(_d))
     _sig(L, "ts_tables_sweep_stats", _i, _vp, _vp, _vp, _vp)
+    _sig(L, "ts_tables_round_log", _i, _vp, _i, _vp, _vp)
     _sig(L, "ts_tables_release", None, _vp)
     for name, res, args in _EXTRA:
         if hasattr(L, name):

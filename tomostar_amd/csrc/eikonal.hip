@@ -348,6 +348,87 @@ __device__ __forceinline__ void load_node(const FastArgs<R> &A, int dir, uint32_
     }
 }
 
+// The new time of the node L describes for this lane's field, from what load_node read: Relax's
+// arithmetic (FP64: bit for bit; FP32: the correctly rounded offset form) or, with kLean, the mixed
+// form's (godunov_lean). on: the lane's field relaxes and does not hold the node fixed (else the
+// result is the current time).
+template <typename R, bool kLean, bool kSecond>
+__device__ __forceinline__ R node_update(const FastArgs<R> &A, const NodeLoad<R> &L, uint64_t act, uint64_t ph,
+                                         int lane, bool &on)
+{
+    constexpr bool any_second = kSecond;  // some field of the wave in its second-order phase
+    const R wa[3][2] = {{L.wlon0, L.wlon1}, {L.wlat0, L.wlat1}, {A.wr0, A.wr1}};
+    on = ((act & ~L.fixed) >> lane) & 1;
+    const R s = L.s;
+    if (!on) return L.cur;
+    const bool second = (ph >> lane) & 1;
+    R first[3];
+    bool minus[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        // the smaller neighbour of the axis
+        minus[ax] = L.mn[ax] <= L.pl[ax];
+        first[ax] = minus[ax] ? L.mn[ax] : L.pl[ax];
+    }
+    // second order: the next node beyond the smaller neighbour
+    auto beyond = [&](R t2[3]) {
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++)
+            t2[ax] = (second && isfinite(first[ax])) ? (minus[ax] ? L.m2[ax] : L.p2[ax]) : (R)INFINITY;
+    };
+    R next, t2[3];
+    if constexpr (kLean && sizeof(R) == 4) {
+        beyond(t2);
+        next = (s > 0.0f) ? godunov_lean32(s, first, t2, wa) : (R)INFINITY;
+    } else if constexpr (kLean) {
+        beyond(t2);
+        next = (s > 0.0) ? godunov_lean64(s, first, t2, wa) : (R)INFINITY;
+    } else if constexpr (sizeof(R) == 4) {
+        // no field of the wave in its second-order phase (wave-uniform): the first-order arithmetic
+        if (!any_second) next = (s > 0.0f) ? godunov_offset_first(s, first, wa) : (R)INFINITY;
+        else {
+            beyond(t2);
+            next = (s > 0.0f) ? godunov_offset(s, first, t2, wa) : (R)INFINITY;
+        }
+    } else {
+        beyond(t2);
+        next = (s > 0.0) ? godunov_abs_w<double>(s, first, t2, wa) : (R)INFINITY;
+    }
+    return next;
+}
+
+// A lane's drop of the node's time: `dropped` wakes its readers (more than the wake tolerance),
+// `moved` counts for the settle test (more than the settle tolerance; FP32 times: and more than the
+// settle threshold in ulps of the time).
+template <typename R>
+__device__ __forceinline__ void drop_tests(const FastArgs<R> &A, R cur, R next, bool &dropped, bool &moved)
+{
+    dropped = !isfinite(cur) || (double)(cur - next) > A.wake_tol;
+    if constexpr (sizeof(R) == 4)
+        moved = !isfinite(cur) || (cur - next) > fmaxf((float)kSettleTol, A.settle_ulps * 1.1920929e-7f * cur);
+    else
+        moved = !isfinite(cur) || (double)(cur - next) > kSettleTol;
+}
+
+// Raise the dirty flags of the nodes that read node (i, j, k) of group g (the wave's fields dropped
+// it): one and two steps on each axis (lanes 0-11, wake_offset) when a field of the group is in its
+// second-order phase, else one step - a first-order update reads no node two steps away, and a field
+// entering its second-order phase wakes every node. Bit 4a + w of the wave-uniform mask: the lane's
+// node lies in the grid.
+template <typename R, bool kSecond>
+__device__ __forceinline__ void wake_readers(const FastArgs<R> &A, size_t gn, int node, int i, int j, int k,
+                                             int lane, int wake_off)
+{
+    const int pos[3] = {i, j, k}, n[3] = {A.nx, A.ny, A.nz};
+    uint32_t in = 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++)
+        in |= (uint32_t)((pos[ax] >= 1) | (pos[ax] >= 2) << 1 | (pos[ax] + 1 < n[ax]) << 2 |
+                         (pos[ax] + 2 < n[ax]) << 3) << (4 * ax);
+    if (!kSecond) in &= 0x555u;
+    if (((uint64_t)in >> lane) & 1) A.dirty[gn + node + wake_off] = 1;
+}
+
 // The update itself, from what load_node read: the new time, and the node's readers woken when it
 // dropped by more than the wake tolerance. FP32 times: returns the lanes whose field moved the node by
 // more than the settle tolerance (the caller gathers them for the group's changed mask).
@@ -357,75 +438,18 @@ template <typename R, bool kCount, bool kLean, bool kSecond>
 __device__ __forceinline__ uint64_t finish_node(const FastArgs<R> &A, const NodeLoad<R> &L, int g, uint64_t act,
                                        uint64_t ph, int lane, unsigned counter, int wake_off)
 {
-    constexpr bool any_second = kSecond;  // some field of the wave in its second-order phase
     if (L.fl == 0) return 0;
-    const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int node = L.node, i = L.i, j = L.j, k = L.k;
+    const int node = L.node;
     const size_t gn = (size_t)g * A.N;
     if (lane == 0) A.dirty[gn + node] = 0;
-    const R wa[3][2] = {{L.wlon0, L.wlon1}, {L.wlat0, L.wlat1}, {A.wr0, A.wr1}};
-    const bool on = ((act & ~L.fixed) >> lane) & 1;
     if (kCount && lane == 0) atomicAdd(&A.evals[counter & 1023], (unsigned long long)__popcll(act & ~L.fixed));
-    const R s = L.s, cur = L.cur;
-    bool dropped = false, moved = false;
-    if (on) {
-        const bool second = (ph >> lane) & 1;
-        R first[3];
-        bool minus[3];
-#pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
-            // the smaller neighbour of the axis
-            minus[ax] = L.mn[ax] <= L.pl[ax];
-            first[ax] = minus[ax] ? L.mn[ax] : L.pl[ax];
-        }
-        // second order: the next node beyond the smaller neighbour
-        auto beyond = [&](R t2[3]) {
-#pragma unroll
-            for (int ax = 0; ax < 3; ax++)
-                t2[ax] = (second && isfinite(first[ax])) ? (minus[ax] ? L.m2[ax] : L.p2[ax]) : (R)INFINITY;
-        };
-        R next, t2[3];
-        if constexpr (kLean && sizeof(R) == 4) {
-            beyond(t2);
-            next = (s > 0.0f) ? godunov_lean32(s, first, t2, wa) : (R)INFINITY;
-        } else if constexpr (kLean) {
-            beyond(t2);
-            next = (s > 0.0) ? godunov_lean64(s, first, t2, wa) : (R)INFINITY;
-        } else if constexpr (sizeof(R) == 4) {
-            // no field of the wave in its second-order phase (wave-uniform): the first-order arithmetic
-            if (!any_second) next = (s > 0.0f) ? godunov_offset_first(s, first, wa) : (R)INFINITY;
-            else {
-                beyond(t2);
-                next = (s > 0.0f) ? godunov_offset(s, first, t2, wa) : (R)INFINITY;
-            }
-        } else {
-            beyond(t2);
-            next = (s > 0.0) ? godunov_abs_w<double>(s, first, t2, wa) : (R)INFINITY;
-        }
-        if (next < cur) {
-            A.T[(gn + node) * kLanes + lane] = next;
-            dropped = !isfinite(cur) || (double)(cur - next) > A.wake_tol;
-            if constexpr (sizeof(R) == 4)
-                moved = !isfinite(cur) || (cur - next) > fmaxf((float)kSettleTol, A.settle_ulps * 1.1920929e-7f * cur);
-            else
-                moved = !isfinite(cur) || (double)(cur - next) > kSettleTol;
-        }
+    bool on, dropped = false, moved = false;
+    const R next = node_update<R, kLean, kSecond>(A, L, act, ph, lane, on);
+    if (on && next < L.cur) {
+        A.T[(gn + node) * kLanes + lane] = next;
+        drop_tests(A, L.cur, next, dropped, moved);
     }
-    const uint64_t woke = __ballot(dropped);
-    if (woke) {
-        // Wake the nodes that read this one: one and two steps on each axis (lanes 0-11, wake_offset)
-        // when a field of the group is in its second-order phase, else one step - a first-order update
-        // reads no node two steps away, and a field entering its second-order phase wakes every node.
-        // Bit 4a + w of the wave-uniform mask: the lane's node lies in the grid.
-        const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz};
-        uint32_t in = 0;
-#pragma unroll
-        for (int ax = 0; ax < 3; ax++)
-            in |= (uint32_t)((pos[ax] >= 1) | (pos[ax] >= 2) << 1 | (pos[ax] + 1 < n[ax]) << 2 |
-                             (pos[ax] + 2 < n[ax]) << 3) << (4 * ax);
-        if (!any_second) in &= 0x555u;
-        if (((uint64_t)in >> lane) & 1) A.dirty[gn + node + wake_off] = 1;
-    }
+    if (__ballot(dropped)) wake_readers<R, kSecond>(A, gn, node, L.i, L.j, L.k, lane, wake_off);
     const uint64_t mv = __ballot(moved);
     if constexpr (sizeof(R) == 8) {
         // FP64 times: into the group's changed mask at once, when the mask (read at the device's scope)
@@ -437,6 +461,29 @@ __device__ __forceinline__ uint64_t finish_node(const FastArgs<R> &A, const Node
         return 0;
     }
     return mv;
+}
+
+// The update of a streaming sweep's node: as finish_node, with the new time returned (the current
+// one when no lane lowers it) for the tile's ring, the moved lanes left to the caller, and the lanes
+// that woke the node's readers returned.
+template <typename R, bool kLean, bool kSecond>
+__device__ __forceinline__ uint64_t stream_update(const FastArgs<R> &A, const NodeLoad<R> &L, int g, uint64_t act,
+                                                  uint64_t ph, int lane, int wake_off, R &next, bool &moved)
+{
+    const size_t gn = (size_t)g * A.N;
+    if (lane == 0) A.dirty[gn + L.node] = 0;
+    bool on, dropped = false;
+    moved = false;
+    next = node_update<R, kLean, kSecond>(A, L, act, ph, lane, on);
+    if (on && next < L.cur) {
+        A.T[(gn + L.node) * kLanes + lane] = next;
+        drop_tests(A, L.cur, next, dropped, moved);
+    } else {
+        next = L.cur;
+    }
+    const uint64_t woke = __ballot(dropped);
+    if (woke) wake_readers<R, kSecond>(A, gn, L.node, L.i, L.j, L.k, lane, wake_off);
+    return woke;
 }
 
 // The lanes a wave saw move, into the group's changed mask: at most one atomic per wave, at its end, and
@@ -458,6 +505,12 @@ template <typename R, bool kLean>
 constexpr bool kSecondOnly = kLean && sizeof(R) == 8;
 template <typename R>
 constexpr bool kFirstOnly = sizeof(R) == 4;
+
+}  // namespace ts
+
+#include "eikonal_stream.hpp"
+
+namespace ts {
 
 // The nodes a wave of eik_fast_plane updates: plane nodes a, a + stride, ... (kSecond: some field of
 // the group in its second-order phase, so the loads two nodes away are issued).
@@ -726,6 +779,8 @@ struct ts_tableset {
     double sweep_by_width[2] = {0, 0};  // sweep-graph device seconds on FP32 / FP64 times
     int32_t rounds_by_width[2] = {0, 0};  // rounds run on FP32 / FP64 times
     int32_t plane_launches_per_round = 0;
+    std::vector<int32_t> round_kind;      // per round of the last solve: 16 for FP64 times | round order
+    std::vector<double> round_seconds;    // and its device seconds
     // fast-form working storage, kept across solves
     dbuf<float> T32;
     dbuf<double> T64;
@@ -738,19 +793,27 @@ struct ts_tableset {
     float settle_ulps = 4096;             // the running solve's FP32 settle threshold (ulps of the time)
     // launch-shape knobs, read once when the table set is created (graphs captured with them stay valid)
     int knob_group_chunk = 0, knob_plane_blocks = 0;
-    hipGraphExec_t graphs[2][2][3][2] = {};  // [FP64][counting][flag reads][wake_tol > 0]
-    double graph_wake_tol[2][2][3][2] = {};  // the wake_tol each graph was captured with
-    bool graph_lean[2][2][3][2] = {};        // and its arithmetic
-    float graph_ulps[2][2][3][2] = {};       // and its FP32 settle threshold
+    hipGraphExec_t graphs[2][2][4][2] = {};  // [FP64][counting][round order][wake_tol > 0]
+    double graph_wake_tol[2][2][4][2] = {};  // the wake_tol each graph was captured with
+    bool graph_lean[2][2][4][2] = {};        // and its arithmetic
+    float graph_ulps[2][2][4][2] = {};       // and its FP32 settle threshold
     // chunks of groups sweeping side by side, each on its own stream with its own graph of plane launches
     static constexpr int kMaxStreams = 8;
     hipStream_t gstream[kMaxStreams] = {};
     hipEvent_t gfork = nullptr, gjoin[kMaxStreams] = {};
-    hipGraphExec_t cgraphs[2][2][3][2][kMaxStreams] = {};
-    double cgraph_wake_tol[2][2][3][2][kMaxStreams] = {};
-    bool cgraph_lean[2][2][3][2][kMaxStreams] = {};
-    float cgraph_ulps[2][2][3][2][kMaxStreams] = {};
-    int cgraph_groups[2][2][3][2][kMaxStreams][2] = {};  // the groups [first, first + count) each swept
+    hipGraphExec_t cgraphs[2][2][4][2][kMaxStreams] = {};
+    double cgraph_wake_tol[2][2][4][2][kMaxStreams] = {};
+    bool cgraph_lean[2][2][4][2][kMaxStreams] = {};
+    float cgraph_ulps[2][2][4][2][kMaxStreams] = {};
+    int cgraph_groups[2][2][4][2][kMaxStreams][2] = {};  // the groups [first, first + count) each swept
+    // streaming rounds, per width (FP32 / FP64 times): tile shape, the task list of every group's tiles
+    // in an order where each tile follows its upwind tiles, and the queues and progress of a round
+    struct StreamPlan {
+        int tu = 0, tv = 0, ntu = 0, ntv = 0, ntasks = 0, slots = 0, grid = 0;
+        dbuf<uint32_t> tasks;
+    } splan[2];
+    dbuf<int> stream_queue, stream_prog;
+    int stream_spin_limit = 1 << 18;
     std::mutex gate;
     ~ts_tableset()
     {
@@ -936,6 +999,7 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
 
 constexpr int kWPB = 4;
 constexpr int kSparse = 2;  // round order after kFlagWithLoads, kFlagFirst: kSparseNodes flags per wave, then the dirty nodes
+constexpr int kStream = 3;  // the streaming sweep: one persistent launch per ordering over column tiles
 #ifndef TS_SPARSE_NODES32
 #define TS_SPARSE_NODES32 4
 #endif
@@ -965,6 +1029,9 @@ int group_chunk(const ts_tableset &t, size_t width)
 // The launch-shape knobs of a new table set (TOMOSTAR_GROUP_CHUNK, TOMOSTAR_PLANE_BLOCKS), read once.
 void read_launch_knobs(ts_tableset &t)
 {
+    // polls of a streaming tile's wait for an upwind tile before it raises the error flag (a wait
+    // normally ends within a few planes' time; each poll takes about a microsecond)
+    if (const char *e = getenv("TOMOSTAR_STREAM_SPIN")) t.stream_spin_limit = std::max(0, atoi(e));
     if (const char *e = getenv("TOMOSTAR_GROUP_CHUNK")) t.knob_group_chunk = std::max(1, atoi(e));
     if (const char *e = getenv("TOMOSTAR_PLANE_BLOCKS")) t.knob_plane_blocks = std::max(1, atoi(e));
 }
@@ -1007,6 +1074,117 @@ void round_end(ts_tableset &t, hipStream_t st, int hold)
                                               sweep_round_cap(t.grid), t.status.p, t.reset.p, hold);
     dim3 rg(blocks_for(t.N, 256), groups);
     eik_fast_reset_dirty<<<rg, 256, 0, st>>>(t.dirty.p, t.reset.p, t.N);
+}
+
+// Tile shapes of the streaming sweep: TU x TV z-columns over NW waves, each column a ring of
+// kStreamRing entries of 64 fields in LDS (FP32 times: 8 x 4 columns, 48 KB; FP64 times: 4 x 4, 48 KB).
+template <typename R>
+struct StreamShape {
+    static constexpr int TU = sizeof(R) == 4 ? 8 : 4, TV = 4, NW = sizeof(R) == 4 ? 8 : 4;
+};
+template <typename R, int TU, int TV>
+constexpr size_t stream_lds_bytes()
+{
+    return (size_t)TU * TV * kStreamRing * (kLanes * sizeof(R) + 1);
+}
+
+// The task list of the streaming rounds on FP32 (w = 0) or FP64 (w = 1) times: every group's tiles, by
+// tile diagonal, then group, then tile, so each tile follows its upwind tiles (one column to the left
+// and one below in the ordering's mirrored coordinates), and the progress slots of a round.
+void stream_plan(ts_tableset &t, int w, int TU, int TV)
+{
+    auto &pl = t.splan[w];
+    const int groups = (t.nfields + kLanes - 1) / kLanes;
+    const int ntu = (t.grid.nx + TU - 1) / TU, ntv = (t.grid.ny + TV - 1) / TV;
+    if (ntu >= 4096 || ntv >= 4096 || groups > 255) fail(TS_DECLINED, "streaming tiles: grid or groups too large");
+    std::vector<uint32_t> list;
+    list.reserve((size_t)groups * ntu * ntv);
+    for (int d = 0; d <= ntu - 1 + ntv - 1; d++)
+        for (int g = 0; g < groups; g++)
+            for (int tu = std::max(0, d - (ntv - 1)); tu <= std::min(ntu - 1, d); tu++)
+                list.push_back((uint32_t)tu | (uint32_t)(d - tu) << 12 | (uint32_t)g << 24);
+    pl.tu = TU;
+    pl.tv = TV;
+    pl.ntu = ntu;
+    pl.ntv = ntv;
+    pl.slots = groups * ntu * ntv;
+    pl.ntasks = (int)list.size();
+    pl.tasks.alloc(list.size());
+    TS_HIP(hipMemcpy(pl.tasks.p, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+}
+
+// Workgroups of one streaming launch per CU: as many as stay resident (a persistent grid). Queried once
+// per kernel (host calls only, none on a stream, so the first launch may be under capture).
+template <typename K>
+int stream_blocks_per_cu(K kernel, int threads, size_t lds)
+{
+    TS_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    TS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+    return std::max(1, per_cu);
+}
+
+template <typename R, int DIR, bool kCount, bool kLean>
+int stream_per_cu_dir()
+{
+    using Sh = StreamShape<R>;
+    static const int per_cu = stream_blocks_per_cu(eik_stream<R, Sh::TU, Sh::TV, Sh::NW, DIR, kCount, kLean>, 64 * Sh::NW,
+                                                   stream_lds_bytes<R, Sh::TU, Sh::TV>());
+    return per_cu;
+}
+
+// The resident workgroups per CU of the eight launches of a streaming round, queried before any round
+// is captured (the queries are host calls, kept out of stream capture).
+template <typename R, bool kCount, bool kLean>
+void stream_prepare()
+{
+    if constexpr (sizeof(R) == 4 && kLean) return;  // not built: such rounds sweep planes
+    (void)stream_per_cu_dir<R, 0, kCount, kLean>(); (void)stream_per_cu_dir<R, 1, kCount, kLean>();
+    (void)stream_per_cu_dir<R, 2, kCount, kLean>(); (void)stream_per_cu_dir<R, 3, kCount, kLean>();
+    (void)stream_per_cu_dir<R, 4, kCount, kLean>(); (void)stream_per_cu_dir<R, 5, kCount, kLean>();
+    (void)stream_per_cu_dir<R, 6, kCount, kLean>(); (void)stream_per_cu_dir<R, 7, kCount, kLean>();
+}
+
+template <typename R, int DIR, bool kCount, bool kLean>
+void launch_stream_dir(ts_tableset &t, const StreamArgs<R> &S, hipStream_t st)
+{
+    using Sh = StreamShape<R>;
+    constexpr size_t lds = stream_lds_bytes<R, Sh::TU, Sh::TV>();
+    const int grid = std::max(1, std::min(S.ntasks, stream_per_cu_dir<R, DIR, kCount, kLean>() * t.ctx->cu_count));
+    eik_stream<R, Sh::TU, Sh::TV, Sh::NW, DIR, kCount, kLean><<<grid, 64 * Sh::NW, lds, st>>>(S);
+}
+
+// One round of the streaming sweep: queues and progress cleared by a kernel (no memset node in a
+// captured graph), the eight orderings as one persistent launch each, then the per-field bookkeeping.
+template <typename R, bool kCount, bool kLean>
+void launch_stream_round(ts_tableset &t, R *T, hipStream_t st, int hold)
+{
+    auto &pl = t.splan[sizeof(R) == 8];
+    StreamArgs<R> S;
+    S.A = fast_args<R>(t, T);
+    S.tasks = pl.tasks.p;
+    S.ntasks = pl.ntasks;
+    S.g0 = 0;
+    S.ntu = pl.ntu;
+    S.ntv = pl.ntv;
+    S.slots = pl.slots;
+    S.spin_limit = t.stream_spin_limit;
+    S.sync.queue = t.stream_queue.p;
+    S.sync.prog = t.stream_prog.p;
+    S.sync.err = t.status.p + 2;
+    const int n = 8 * pl.slots;
+    eik_stream_reset<<<blocks_for(std::max(n, 8), 256), 256, 0, st>>>(t.stream_queue.p, t.stream_prog.p, n);
+    launch_stream_dir<R, 0, kCount, kLean>(t, S, st);
+    launch_stream_dir<R, 1, kCount, kLean>(t, S, st);
+    launch_stream_dir<R, 2, kCount, kLean>(t, S, st);
+    launch_stream_dir<R, 3, kCount, kLean>(t, S, st);
+    launch_stream_dir<R, 4, kCount, kLean>(t, S, st);
+    launch_stream_dir<R, 5, kCount, kLean>(t, S, st);
+    launch_stream_dir<R, 6, kCount, kLean>(t, S, st);
+    launch_stream_dir<R, 7, kCount, kLean>(t, S, st);
+    TS_HIP(hipGetLastError());
+    t.plane_launches_per_round = 8;
+    round_end(t, st, hold);
 }
 
 template <typename R, bool kCount, int kOrder, bool kLean>
@@ -1061,7 +1239,12 @@ int group_streams(int groups, bool graphs)
 template <typename R, bool kCount, bool kLean>
 void launch_round_order(ts_tableset &t, R *T, hipStream_t st, int hold, int order)
 {
-    if (order == kFlagWithLoads) launch_round<R, kCount, kFlagWithLoads, kLean>(t, T, st, hold);
+    if (order == kStream) {
+        // the streaming kernels are built for the arithmetic each width runs by default and for
+        // the FP64 form's (FP32 times: the correctly rounded offset form)
+        if constexpr (sizeof(R) == 4 && kLean) launch_round<R, kCount, kFlagFirst, kLean>(t, T, st, hold);
+        else launch_stream_round<R, kCount, kLean>(t, T, st, hold);
+    } else if (order == kFlagWithLoads) launch_round<R, kCount, kFlagWithLoads, kLean>(t, T, st, hold);
     else if (order == kFlagFirst) launch_round<R, kCount, kFlagFirst, kLean>(t, T, st, hold);
     else launch_round<R, kCount, kSparse, kLean>(t, T, st, hold);
 }
@@ -1098,8 +1281,13 @@ void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count,
 // 4; 64 flags per wave from a late round on is slower in every round tried (FP64 rounds 2, 3, 4: 1.103,
 // 0.761, 0.584 s against 0.569 s; FP32 rounds 1, 2: 0.617, 0.407 s against 0.226 s) - the late rounds
 // still hold dirty nodes enough that a wave working through 64 flags' worth becomes the launch's tail.
+// - rounds before TOMOSTAR_STREAM_ROUNDS (a run from the seeds) or TOMOSTAR_STREAM_ROUNDS2 (the mixed
+//   form's second-order phase) run the streaming sweep.
 int round_order(bool all_dirty, bool one_phase, int round)
 {
+    int stream = 0;
+    if (const char *e = getenv(all_dirty ? "TOMOSTAR_STREAM_ROUNDS2" : "TOMOSTAR_STREAM_ROUNDS")) stream = std::max(0, atoi(e));
+    if (round < stream) return kStream;
     int first = 0;
     if (all_dirty) {
         first = 1;
@@ -1206,9 +1394,9 @@ void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel
     }
     for (int round = 0; round < max_rounds; round++) {
         if (cancelled(cancel)) fail(TS_CANCELLED, "cancelled");
-        TS_HIP(hipMemsetAsync(t.status.p, 0, 2 * sizeof(int32_t), st));
+        TS_HIP(hipMemsetAsync(t.status.p, 0, 3 * sizeof(int32_t), st));
         const int order = round_order(all_dirty, one_phase, round);
-        if (S > 1) {
+        if (S > 1 && order != kStream) {
             // chunk 0 sweeps on the solve's stream, every other chunk on its own stream between a fork
             // and a join on the solve's stream
             std::vector<hipGraphExec_t> cg(S, nullptr);
@@ -1235,13 +1423,16 @@ void run_rounds(ts_tableset &t, R *T, bool count, const volatile int32_t *cancel
             else launch_round_as<R>(t, T, st, hold, count, order);
         }
         TS_HIP(hipEventRecord(ev1, st));
-        TS_HIP(hipMemcpyAsync(h_status, t.status.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        TS_HIP(hipMemcpyAsync(h_status, t.status.p, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
         TS_HIP(hipStreamSynchronize(st));
         float ms = 0;
         TS_HIP(hipEventElapsedTime(&ms, ev0, ev1));
         t.sweep_seconds += ms * 1e-3;
+        t.round_kind.push_back((sizeof(R) == 8 ? 16 : 0) | order);
+        t.round_seconds.push_back(ms * 1e-3);
         t.sweep_by_width[sizeof(R) == 8] += ms * 1e-3;
         t.rounds_by_width[sizeof(R) == 8]++;
+        if (h_status[2] > 0) fail(TS_EDEVICE, "streaming sweep: a tile waited for its upwind tiles beyond the spin limit");
         if (h_status[1] > 0) fail(TS_DECLINED, "sweeping not settled after %d rounds", sweep_round_cap(t.grid));
         if (h_status[0] == 0) return;
     }
@@ -1279,8 +1470,21 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     t.dirty.ensure(nodes_total);
     t.seedmask.ensure(nodes_total);
     t.active.ensure(groups); t.phase1.ensure(groups); t.changed.ensure(groups); t.reset.ensure(groups);
-    t.model_dev.ensure(slots); t.state.ensure(slots); t.rounds_dev.ensure(2 * slots); t.status.ensure(2);
+    t.model_dev.ensure(slots); t.state.ensure(slots); t.rounds_dev.ensure(2 * slots); t.status.ensure(3);
     t.evals.ensure(2048);
+    // the streaming kernels' launch shapes, outside any stream capture
+    if (count) {
+        stream_prepare<float, true, false>(); stream_prepare<double, true, false>(); stream_prepare<double, true, true>();
+    } else {
+        stream_prepare<float, false, false>(); stream_prepare<double, false, false>(); stream_prepare<double, false, true>();
+    }
+    // the streaming rounds' task lists (once per table set: captured graphs keep these buffers)
+    if (!t.splan[0].ntasks) {
+        stream_plan(t, 0, StreamShape<float>::TU, StreamShape<float>::TV);
+        stream_plan(t, 1, StreamShape<double>::TU, StreamShape<double>::TV);
+        t.stream_queue.alloc(8);
+        t.stream_prog.alloc((size_t)8 * std::max(t.splan[0].slots, t.splan[1].slots));
+    }
     t.tab32.ensure((size_t)t.nfields * N);
     t.tab64.ensure((size_t)t.nfields * N);
 
@@ -1326,9 +1530,11 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     TS_HIP(hipGetLastError());
     TS_HIP(hipStreamSynchronize(st));
     int32_t *h_status = nullptr;
-    TS_HIP(hipHostMalloc((void **)&h_status, 2 * sizeof(int32_t), hipHostMallocDefault));
+    TS_HIP(hipHostMalloc((void **)&h_status, 3 * sizeof(int32_t), hipHostMallocDefault));
     struct Free { int32_t *p; ~Free() { if (p) (void)hipHostFree(p); } } free_h{h_status};
     t.sweep_seconds = 0;
+    t.round_kind.clear();
+    t.round_seconds.clear();
     t.sweep_by_width[0] = t.sweep_by_width[1] = 0;
     t.rounds_by_width[0] = t.rounds_by_width[1] = 0;
     // The FP64 form wakes a node's readers on every drop (Relax's result, bit for bit), and so does the
@@ -1655,6 +1861,17 @@ int ts_tables_stats(ts_tableset *t, double *evaluations_fp32, double *evaluation
     if (seconds) *seconds = t->seconds;
     if (sweep_seconds) *sweep_seconds = t->sweep_seconds;
     return TS_OK;
+}
+
+int ts_tables_round_log(ts_tableset *t, int max, int32_t *kind, double *seconds)
+{
+    if (!t || max < 0 || (max > 0 && (!kind || !seconds))) return TS_EINVAL;
+    const int n = (int)std::min<size_t>((size_t)max, t->round_kind.size());
+    for (int q = 0; q < n; q++) {
+        kind[q] = t->round_kind[q];
+        seconds[q] = t->round_seconds[q];
+    }
+    return (int)t->round_kind.size();
 }
 
 int ts_tables_sweep_stats(ts_tableset *t, int32_t rounds[2], int32_t *plane_launches_per_round, double seconds[2])

@@ -169,6 +169,16 @@ class TableSet:
                 "plane_launches_per_round": per_round.value,
                 "sweep_seconds_fp32": float(secs[0]), "sweep_seconds_fp64": float(secs[1])}
 
+    def round_log(self):
+        """[(width, order, seconds)] per round of the last fast solve: width 32 or 64 (the times swept),
+        order 0 (flag with every load), 1 (flag first), 2 (a few flags per wave), 3 (streaming tiles)."""
+        L = _lib.load()
+        n = L.ts_tables_round_log(self._h, 0, None, None)
+        check(min(n, 0))
+        kind, secs = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1))
+        L.ts_tables_round_log(self._h, n, _lib.ptr(kind), _lib.ptr(secs))
+        return [(64 if k & 16 else 32, int(k & 15), float(x)) for k, x in zip(kind[:n], secs[:n])]
+
     def release(self):
         if self._h is not None:
             _lib.load().ts_tables_release(self._h)
