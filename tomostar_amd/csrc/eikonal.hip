@@ -463,26 +463,60 @@ __device__ __forceinline__ uint64_t finish_node(const FastArgs<R> &A, const Node
     return mv;
 }
 
+// Stores and loads coherent across the device's L2 caches (write-through / miss-forcing, "sc1"): the
+// data a streaming tile exchanges with the tiles beside it within one launch. Everything else a tile
+// touches is its own, or was written by an earlier launch.
+template <typename T>
+__device__ inline void st_dev(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline T ld_dev(T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The update of a streaming sweep's node: as finish_node, with the new time returned (the current
 // one when no lane lowers it) for the tile's ring, the moved lanes left to the caller, and the lanes
-// that woke the node's readers returned.
+// that woke the node's readers returned. dev (wave-uniform): the node lies within two columns of the
+// tile's edge, so its time, flag and wakes are stored device-coherent (tiles beside it read or
+// write them within the launch).
 template <typename R, bool kLean, bool kSecond>
 __device__ __forceinline__ uint64_t stream_update(const FastArgs<R> &A, const NodeLoad<R> &L, int g, uint64_t act,
-                                                  uint64_t ph, int lane, int wake_off, R &next, bool &moved)
+                                                  uint64_t ph, int lane, int wake_off, bool dev, R &next, bool &moved)
 {
     const size_t gn = (size_t)g * A.N;
-    if (lane == 0) A.dirty[gn + L.node] = 0;
+    if (lane == 0) {
+        if (dev) st_dev(&A.dirty[gn + L.node], (uint8_t)0);
+        else A.dirty[gn + L.node] = 0;
+    }
     bool on, dropped = false;
     moved = false;
     next = node_update<R, kLean, kSecond>(A, L, act, ph, lane, on);
     if (on && next < L.cur) {
-        A.T[(gn + L.node) * kLanes + lane] = next;
+        R *at = &A.T[(gn + L.node) * kLanes + lane];
+        if (dev) st_dev(at, next);
+        else *at = next;
         drop_tests(A, L.cur, next, dropped, moved);
     } else {
         next = L.cur;
     }
     const uint64_t woke = __ballot(dropped);
-    if (woke) wake_readers<R, kSecond>(A, gn, L.node, L.i, L.j, L.k, lane, wake_off);
+    if (woke) {
+        const int pos[3] = {L.i, L.j, L.k}, n[3] = {A.nx, A.ny, A.nz};
+        uint32_t in = 0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++)
+            in |= (uint32_t)((pos[ax] >= 1) | (pos[ax] >= 2) << 1 | (pos[ax] + 1 < n[ax]) << 2 |
+                             (pos[ax] + 2 < n[ax]) << 3) << (4 * ax);
+        if (!kSecond) in &= 0x555u;
+        if (((uint64_t)in >> lane) & 1) {
+            uint8_t *f = &A.dirty[gn + L.node + wake_off];
+            if (dev) st_dev(f, (uint8_t)1);
+            else *f = 1;
+        }
+    }
     return woke;
 }
 
@@ -814,6 +848,8 @@ struct ts_tableset {
     } splan[2];
     dbuf<int> stream_queue, stream_prog;
     int stream_spin_limit = 1 << 18;
+    int stream_shape[2] = {0, 0};         // tile shape per width (with_shape)
+    int stream_publish = 1;               // planes a streaming tile completes between two publications
     std::mutex gate;
     ~ts_tableset()
     {
@@ -1000,6 +1036,7 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
 constexpr int kWPB = 4;
 constexpr int kSparse = 2;  // round order after kFlagWithLoads, kFlagFirst: kSparseNodes flags per wave, then the dirty nodes
 constexpr int kStream = 3;  // the streaming sweep: one persistent launch per ordering over column tiles
+constexpr int kStreamShapes = 2;  // tile shapes per width (with_shape)
 #ifndef TS_SPARSE_NODES32
 #define TS_SPARSE_NODES32 4
 #endif
@@ -1032,6 +1069,9 @@ void read_launch_knobs(ts_tableset &t)
     // polls of a streaming tile's wait for an upwind tile before it raises the error flag (a wait
     // normally ends within a few planes' time; each poll takes about a microsecond)
     if (const char *e = getenv("TOMOSTAR_STREAM_SPIN")) t.stream_spin_limit = std::max(0, atoi(e));
+    if (const char *e = getenv("TOMOSTAR_STREAM_SHAPE32")) t.stream_shape[0] = std::clamp(atoi(e), 0, kStreamShapes - 1);
+    if (const char *e = getenv("TOMOSTAR_STREAM_SHAPE64")) t.stream_shape[1] = std::clamp(atoi(e), 0, kStreamShapes - 1);
+    if (const char *e = getenv("TOMOSTAR_STREAM_PUBLISH")) t.stream_publish = std::clamp(atoi(e), 1, 64);
     if (const char *e = getenv("TOMOSTAR_GROUP_CHUNK")) t.knob_group_chunk = std::max(1, atoi(e));
     if (const char *e = getenv("TOMOSTAR_PLANE_BLOCKS")) t.knob_plane_blocks = std::max(1, atoi(e));
 }
@@ -1077,11 +1117,23 @@ void round_end(ts_tableset &t, hipStream_t st, int hold)
 }
 
 // Tile shapes of the streaming sweep: TU x TV z-columns over NW waves, each column a ring of
-// kStreamRing entries of 64 fields in LDS (FP32 times: 8 x 4 columns, 48 KB; FP64 times: 4 x 4, 48 KB).
-template <typename R>
-struct StreamShape {
-    static constexpr int TU = sizeof(R) == 4 ? 8 : 4, TV = 4, NW = sizeof(R) == 4 ? 8 : 4;
+// kStreamRing entries of 64 fields in LDS (4 x 4 columns of FP64 times or 8 x 4 of FP32: 48 KB). The
+// table set picks one per width (TOMOSTAR_STREAM_SHAPE32 / TOMOSTAR_STREAM_SHAPE64: index below).
+template <int TU_, int TV_, int NW_>
+struct Shp {
+    static constexpr int TU = TU_, TV = TV_, NW = NW_;
 };
+template <typename R, typename F>
+void with_shape(int idx, F &&f)
+{
+    if constexpr (sizeof(R) == 4) {
+        if (idx == 1) f(Shp<8, 4, 16>{});
+        else f(Shp<8, 4, 8>{});
+    } else {
+        if (idx == 1) f(Shp<4, 4, 8>{});
+        else f(Shp<4, 4, 4>{});
+    }
+}
 template <typename R, int TU, int TV>
 constexpr size_t stream_lds_bytes()
 {
@@ -1124,33 +1176,35 @@ int stream_blocks_per_cu(K kernel, int threads, size_t lds)
     return std::max(1, per_cu);
 }
 
-template <typename R, int DIR, bool kCount, bool kLean>
+template <typename R, typename Sh, int DIR, bool kCount, bool kLean>
 int stream_per_cu_dir()
 {
-    using Sh = StreamShape<R>;
-    static const int per_cu = stream_blocks_per_cu(eik_stream<R, Sh::TU, Sh::TV, Sh::NW, DIR, kCount, kLean>, 64 * Sh::NW,
-                                                   stream_lds_bytes<R, Sh::TU, Sh::TV>());
+    static const int per_cu = stream_blocks_per_cu(eik_stream<R, Sh::TU, Sh::TV, Sh::NW, DIR, kCount, kLean>,
+                                                   64 * Sh::NW, stream_lds_bytes<R, Sh::TU, Sh::TV>());
     return per_cu;
 }
 
 // The resident workgroups per CU of the eight launches of a streaming round, queried before any round
 // is captured (the queries are host calls, kept out of stream capture).
 template <typename R, bool kCount, bool kLean>
-void stream_prepare()
+void stream_prepare(int shape)
 {
     if constexpr (sizeof(R) == 4 && kLean) return;  // not built: such rounds sweep planes
-    (void)stream_per_cu_dir<R, 0, kCount, kLean>(); (void)stream_per_cu_dir<R, 1, kCount, kLean>();
-    (void)stream_per_cu_dir<R, 2, kCount, kLean>(); (void)stream_per_cu_dir<R, 3, kCount, kLean>();
-    (void)stream_per_cu_dir<R, 4, kCount, kLean>(); (void)stream_per_cu_dir<R, 5, kCount, kLean>();
-    (void)stream_per_cu_dir<R, 6, kCount, kLean>(); (void)stream_per_cu_dir<R, 7, kCount, kLean>();
+    else
+        with_shape<R>(shape, [](auto sh) {
+            using Sh = decltype(sh);
+            (void)stream_per_cu_dir<R, Sh, 0, kCount, kLean>(); (void)stream_per_cu_dir<R, Sh, 1, kCount, kLean>();
+            (void)stream_per_cu_dir<R, Sh, 2, kCount, kLean>(); (void)stream_per_cu_dir<R, Sh, 3, kCount, kLean>();
+            (void)stream_per_cu_dir<R, Sh, 4, kCount, kLean>(); (void)stream_per_cu_dir<R, Sh, 5, kCount, kLean>();
+            (void)stream_per_cu_dir<R, Sh, 6, kCount, kLean>(); (void)stream_per_cu_dir<R, Sh, 7, kCount, kLean>();
+        });
 }
 
-template <typename R, int DIR, bool kCount, bool kLean>
+template <typename R, typename Sh, int DIR, bool kCount, bool kLean>
 void launch_stream_dir(ts_tableset &t, const StreamArgs<R> &S, hipStream_t st)
 {
-    using Sh = StreamShape<R>;
     constexpr size_t lds = stream_lds_bytes<R, Sh::TU, Sh::TV>();
-    const int grid = std::max(1, std::min(S.ntasks, stream_per_cu_dir<R, DIR, kCount, kLean>() * t.ctx->cu_count));
+    const int grid = std::max(1, std::min(S.ntasks, stream_per_cu_dir<R, Sh, DIR, kCount, kLean>() * t.ctx->cu_count));
     eik_stream<R, Sh::TU, Sh::TV, Sh::NW, DIR, kCount, kLean><<<grid, 64 * Sh::NW, lds, st>>>(S);
 }
 
@@ -1169,19 +1223,23 @@ void launch_stream_round(ts_tableset &t, R *T, hipStream_t st, int hold)
     S.ntv = pl.ntv;
     S.slots = pl.slots;
     S.spin_limit = t.stream_spin_limit;
+    S.publish = t.stream_publish;
     S.sync.queue = t.stream_queue.p;
     S.sync.prog = t.stream_prog.p;
     S.sync.err = t.status.p + 2;
     const int n = 8 * pl.slots;
     eik_stream_reset<<<blocks_for(std::max(n, 8), 256), 256, 0, st>>>(t.stream_queue.p, t.stream_prog.p, n);
-    launch_stream_dir<R, 0, kCount, kLean>(t, S, st);
-    launch_stream_dir<R, 1, kCount, kLean>(t, S, st);
-    launch_stream_dir<R, 2, kCount, kLean>(t, S, st);
-    launch_stream_dir<R, 3, kCount, kLean>(t, S, st);
-    launch_stream_dir<R, 4, kCount, kLean>(t, S, st);
-    launch_stream_dir<R, 5, kCount, kLean>(t, S, st);
-    launch_stream_dir<R, 6, kCount, kLean>(t, S, st);
-    launch_stream_dir<R, 7, kCount, kLean>(t, S, st);
+    with_shape<R>(t.stream_shape[sizeof(R) == 8], [&](auto sh) {
+        using Sh = decltype(sh);
+        launch_stream_dir<R, Sh, 0, kCount, kLean>(t, S, st);
+        launch_stream_dir<R, Sh, 1, kCount, kLean>(t, S, st);
+        launch_stream_dir<R, Sh, 2, kCount, kLean>(t, S, st);
+        launch_stream_dir<R, Sh, 3, kCount, kLean>(t, S, st);
+        launch_stream_dir<R, Sh, 4, kCount, kLean>(t, S, st);
+        launch_stream_dir<R, Sh, 5, kCount, kLean>(t, S, st);
+        launch_stream_dir<R, Sh, 6, kCount, kLean>(t, S, st);
+        launch_stream_dir<R, Sh, 7, kCount, kLean>(t, S, st);
+    });
     TS_HIP(hipGetLastError());
     t.plane_launches_per_round = 8;
     round_end(t, st, hold);
@@ -1473,15 +1531,20 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     t.model_dev.ensure(slots); t.state.ensure(slots); t.rounds_dev.ensure(2 * slots); t.status.ensure(3);
     t.evals.ensure(2048);
     // the streaming kernels' launch shapes, outside any stream capture
-    if (count) {
-        stream_prepare<float, true, false>(); stream_prepare<double, true, false>(); stream_prepare<double, true, true>();
-    } else {
-        stream_prepare<float, false, false>(); stream_prepare<double, false, false>(); stream_prepare<double, false, true>();
+    {
+        const int s32 = t.stream_shape[0], s64 = t.stream_shape[1];
+        if (count) {
+            stream_prepare<float, true, false>(s32); stream_prepare<double, true, false>(s64);
+            stream_prepare<double, true, true>(s64);
+        } else {
+            stream_prepare<float, false, false>(s32); stream_prepare<double, false, false>(s64);
+            stream_prepare<double, false, true>(s64);
+        }
     }
     // the streaming rounds' task lists (once per table set: captured graphs keep these buffers)
     if (!t.splan[0].ntasks) {
-        stream_plan(t, 0, StreamShape<float>::TU, StreamShape<float>::TV);
-        stream_plan(t, 1, StreamShape<double>::TU, StreamShape<double>::TV);
+        with_shape<float>(t.stream_shape[0], [&](auto sh) { stream_plan(t, 0, decltype(sh)::TU, decltype(sh)::TV); });
+        with_shape<double>(t.stream_shape[1], [&](auto sh) { stream_plan(t, 1, decltype(sh)::TU, decltype(sh)::TV); });
         t.stream_queue.alloc(8);
         t.stream_prog.alloc((size_t)8 * std::max(t.splan[0].slots, t.splan[1].slots));
     }

@@ -14,9 +14,13 @@
 //     ahead: old times of the downwind tiles (they do not change until this tile has passed them),
 //     new times of the upwind tiles once those have published that they completed the plane;
 //   - the workgroups take tasks from a queue in an order where every tile follows its upwind tiles
-//     (tile diagonal, group, tile), so a tile only ever waits for a tile held by a running workgroup,
-//     and a tile publishes its completed planes with a release, read by its downwind tiles with an
-//     acquire. A wait that exceeds a spin limit raises an error flag instead of hanging.
+//     (tile diagonal, group, tile), so a tile only ever waits for a tile held by a running workgroup.
+//     What tiles exchange within a launch (the times, flags and wakes of the two columns next to a
+//     tile edge, and the per-tile count of completed planes) is stored and loaded device-coherent
+//     (write-through / miss-forcing), and a wave waits for its stores to complete before its tile
+//     publishes a plane: no L2 write-back or invalidation per plane (agent-scope fences cost ~150 us a
+//     plane here, 20x the plane's work). A wait that exceeds a spin limit raises an error flag
+//     instead of hanging.
 // A node is updated only when something it reads changed since its last update (the dirty flag of
 // the plane kernels: the byte loaded a plane ahead, or, for changes within the last planes of this
 // tile, a per-entry drop bit in LDS), and it raises the dirty flags of its twelve readers as the plane
@@ -46,6 +50,7 @@ struct StreamArgs {
     FastArgs<R> A;
     const uint32_t *tasks;  // tu | tv << 12 | (group - g0) << 24, every tile after its upwind tiles
     int ntasks, g0, ntu, ntv, slots, spin_limit;
+    int publish;            // planes between two publications of a tile's progress (and store waits)
     StreamSync sync;
 };
 
@@ -162,7 +167,8 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<R> &S, R *ring, uin
             L.wlon0 = cs[c].wx0; L.wlon1 = cs[c].wx1; L.wlat0 = cs[c].wy0; L.wlat1 = cs[c].wy1;
             R next;
             bool moved;
-            const uint64_t woke = stream_update<R, kLean, kSecond>(A, L, g, act, ph, lane, wake_off, next, moved);
+            const bool dev = a <= 1 || a >= TU - 2 || b <= 1 || b >= TV - 2;
+            const uint64_t woke = stream_update<R, kLean, kSecond>(A, L, g, act, ph, lane, wake_off, dev, next, moved);
             if (next < cur) ring[((col * RG) + sk) * kLanes + lane] = next;
             if (lane == 0) drop[col * RG + sk] = woke != 0;
             mv |= __ballot(moved);
@@ -177,8 +183,15 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<R> &S, R *ring, uin
             ring[((col * RG) + k3 % RG) * kLanes + lane] = cs[c].pre;
             if (lane == 0) drop[col * RG + k3 % RG] = 0;
         }
-        // (C) the times and flags this wave wrote at plane P are visible before the plane is published
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // (C) the device-coherent stores this wave made up to plane P have completed before the plane is
+        // published (every S.publish planes and at the last; the fences only keep the compiler from
+        // moving memory operations across the wait)
+        const bool pub = (P - Pb + 1) % S.publish == 0 || P == Pe;
+        if (pub) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
         // (D) what plane P + 1 reads beyond the ring: old times first (no wait needed)
 #pragma unroll
         for (int c = 0; c < C; c++) {
@@ -222,7 +235,7 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<R> &S, R *ring, uin
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         };
         if (need_u) wait_for(seen_u, slot - 1);
         if (need_v) wait_for(seen_v, slot - S.ntu);
@@ -232,18 +245,22 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<R> &S, R *ring, uin
             const int kn = P - (u0 + a) - (v0 + b) + 1;
             const int xp = u0 + a, yp = v0 + b;
             if (!real[c] || kn < 0 || kn >= nz) continue;
+            // times the upwind tiles wrote within this launch, and flags they may have raised: read
+            // device-coherent
+            auto tdev = [&](int node) -> R { return ld_dev(&Tg[(size_t)node * kLanes + lane]); };
             if (a <= 1) {
-                if (a == 0) cs[c].hu[0] = xp >= 1 ? tval(node_at(xp - 1, yp, kn)) : inf;
-                if (kSecond) cs[c].hu[1] = xp >= 2 ? tval(node_at(xp - 2, yp, kn)) : inf;
+                if (a == 0) cs[c].hu[0] = xp >= 1 ? tdev(node_at(xp - 1, yp, kn)) : inf;
+                if (kSecond) cs[c].hu[1] = xp >= 2 ? tdev(node_at(xp - 2, yp, kn)) : inf;
             }
             if (b <= 1) {
-                if (b == 0) cs[c].hv[0] = yp >= 1 ? tval(node_at(xp, yp - 1, kn)) : inf;
-                if (kSecond) cs[c].hv[1] = yp >= 2 ? tval(node_at(xp, yp - 2, kn)) : inf;
+                if (b == 0) cs[c].hv[0] = yp >= 1 ? tdev(node_at(xp, yp - 1, kn)) : inf;
+                if (kSecond) cs[c].hv[1] = yp >= 2 ? tdev(node_at(xp, yp - 2, kn)) : inf;
             }
-            cs[c].flag = A.dirty[gn + node_at(xp, yp, kn)];
+            uint8_t *fp = &A.dirty[gn + node_at(xp, yp, kn)];
+            cs[c].flag = (a <= 1 || b <= 1) ? ld_dev(fp) : *fp;
         }
         __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(&prog[slot], P + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (pub && threadIdx.x == 0) __hip_atomic_store(&prog[slot], P + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (kCount && lane == 0 && evals) atomicAdd(&A.evals[blockIdx.x & 1023], evals);
     mark_changed(&A.changed[g], mv, lane);
@@ -286,7 +303,7 @@ __global__ __launch_bounds__(64 * NW) void eik_stream(StreamArgs<R> S)
         }
         // the tile is done: its downwind tiles wait no more
         __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(&prog[slot], INT_MAX, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) __hip_atomic_store(&prog[slot], INT_MAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
     }
 }
