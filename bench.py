@@ -488,9 +488,10 @@ def cpu_baseline_iteration(wl, r):
 
 def uneven_leg(dev, args):
     """The uneven batch: 64 single-field problems on grids from 41 x 41 x 21 to 161 x 161 x 81 at
-    0.5-2 km, swept together (one launch per sweep step over all problems); the tuner's choice per
-    size class; the mixed form against the FP64 form (Relax to the last bit) per problem; the plain
-    FP64 one-problem-per-solve form timed on a sample of the problems."""
+    0.5-2 km, swept together (plane launches over all problems, or one persistent launch per ordering
+    over every problem's column tiles, or the hybrid); the tuner's choices (sweep method, tile width
+    and nodes per thread per size class) by timing; the mixed form against the FP64 form (Relax to the
+    last bit) per problem; the plain FP64 one-problem-per-solve form timed on a sample of the problems."""
     from tomostar_amd import eikonal as E
     from tomostar_amd import workloads as W
     from tomostar_amd.gridbatch import GridBatch
@@ -513,6 +514,8 @@ def uneven_leg(dev, args):
     mixed = [b.read(q) for q in range(len(probs))]
     rounds = b.rounds.copy()
     st = b.stats()
+    sched = b.schedule()
+    round_log = b.round_log()
     t0 = time.perf_counter()
     rc, why = b.solve("fast_fp64")
     t64 = time.perf_counter() - t0
@@ -543,6 +546,11 @@ def uneven_leg(dev, args):
            "nodes": total, "value": round(value, 4), "unit": "Gnode/s (solved nodes per second)",
            "seconds": round(float(np.mean(times)), 3), "fp64_seconds": round(t64, 3),
            "tuner_nodes_per_thread": {"<2e5 nodes": choice[0], "2e5-8e5": choice[1], ">=8e5": choice[2]},
+           "tuner_schedule": {"method": sched["method"],
+                              "tile_width": {"<2e5 nodes": sched["tile_width"][0], "2e5-8e5": sched["tile_width"][1],
+                                             ">=8e5": sched["tile_width"][2]},
+                              "trial_mixed_solve_ms": {k: round(v, 1) for k, v in sched["trial_ms"].items()}},
+           "round_ms": [[w, m, round(x * 1e3, 1)] for w, m, x in round_log],
            "max_rel_error_vs_fp64": max(errs), "rounds_max": [int(rounds[:, 0].max()), int(rounds[:, 1].max())],
            "plain_fp64_gpu": {"value": round(plain, 4), "unit": "Gnode/s", "sample": f"problems {sample}"},
            "speedup_vs_plain_fp64_gpu": round(value / plain, 2)}
@@ -552,7 +560,8 @@ def uneven_leg(dev, args):
     if st["sweep_seconds"] > 0:
         ach = alg / st["sweep_seconds"] / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "mg_plane",
+                           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                           "kernel": {"planes": "mg_plane", "stream": "mg_stream", "hybrid": "mg_plane + mg_stream"}[sched["method"]],
                            "units_per_step": {"fp32_updates": st["updates_fp32"], "fp64_updates": st["updates_fp64"]},
                            "sweep_seconds": round(st["sweep_seconds"], 4)}
     log(f"uneven leg: {out}")

@@ -152,15 +152,31 @@ void ts_tables_release(ts_tableset *t);
  * Many single-field problems, each on its own grid (the adaptive and resolution-test runs, where
  * the reference calls EikonalOpenCl.TrySolve once per grid, EikonalOpenCl.cs:138-343): grids[q],
  * slowness[q] (nodes of grid q), the seeds of problem q at seed_offsets[q] .. seed_offsets[q+1].
- * Every sweep step is one launch over all problems. mode: TS_EIK_FAST_FP64 (Relax to the last bit
+ * Every sweep step is one launch over all problems, or (the streaming method, below) every sweep
+ * ordering one persistent launch over all problems' tiles. mode: TS_EIK_FAST_FP64 (Relax to the last bit
  * per problem) or TS_EIK_FAST_FP32 (first order on FP32 times, second order on FP64 times). */
 int ts_gridbatch_create(ts_context *ctx, int nproblems, const ts_grid *grids, const double *const *slowness,
                         const int64_t *seed_offsets, const int32_t *seed_index, const double *seed_time,
                         ts_gridbatch **out, char *reason, int reason_len);
 /* Per size class of problem (< 2e5, < 8e5, >= 8e5 nodes), how many nodes a thread updates per step
  * (1, 2 or 4), chosen by timing one trial round of each class with each choice; choice[3] receives
- * the picks. No counterpart in the reference: the batch's own tuner. */
+ * the picks. Then the streaming tile width per class and the method (ts_gridbatch_schedule). No
+ * counterpart in the reference: the batch's own tuner. */
 int ts_gridbatch_tune(ts_gridbatch *b, int32_t choice[3]);
+/* The batch's sweep: method 0 (one launch per plane step over every problem), 1 (streaming: one
+ * persistent launch per ordering over a work queue of every problem's column tiles) or 2 (a run's
+ * first round in plane launches, the rounds after it streaming), the streaming tile width (4, 8 or 16
+ * columns along x; 64 / width along y) per size class, the plane method's nodes per thread per class,
+ * and the tuner's trial mixed solve of the batch with each method (ms; 0 before ts_gridbatch_tune).
+ * ts_gridbatch_tune picks all of them by timing; ts_gridbatch_set_schedule forces a method and
+ * (tile_w non-null) the tile widths. No counterpart in the reference: the batch's own schedule. */
+int ts_gridbatch_schedule(ts_gridbatch *b, int32_t *method, int32_t tile_w[3], int32_t nodes_per_thread[3],
+                          double trial_ms[3]);
+int ts_gridbatch_set_schedule(ts_gridbatch *b, int method, const int32_t tile_w[3]);
+/* The rounds of the last solve, in order: kind[q] = 16 for a round on FP64 times (0: FP32) | the method
+ * that swept it, seconds[q] its device seconds. Fills min(max, rounds) entries and returns the number of
+ * rounds. No counterpart in the reference (diagnostics of the device schedule). */
+int ts_gridbatch_round_log(ts_gridbatch *b, int max, int32_t *kind, double *seconds);
 /* rounds_out[2 * nproblems]: rounds of each problem's first- and second-order phases. */
 int ts_gridbatch_solve(ts_gridbatch *b, int mode, const volatile int32_t *cancel, int32_t *rounds_out,
                        char *reason, int reason_len);

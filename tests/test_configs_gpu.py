@@ -56,12 +56,22 @@ def test_uneven_batch_full_size(device, oracle):
     assert min(sizes) < 60_000 and max(sizes) > 1_900_000
     seeds = [E.point_source(p.grid, p.slowness, *p.source) for p in probs]
     b = GridBatch(device, [p.grid for p in probs], [p.slowness for p in probs], seeds)
+    b.set_schedule("planes")
     rc, why = b.solve("fast_fp64")
     assert rc == 0, why
     fp64 = [b.read(q) for q in range(len(probs))]
     rc, why = b.solve("fast_fp32")
     assert rc == 0, why
     mixed = [b.read(q) for q in range(len(probs))]
+    # the streaming sweep (one persistent launch per ordering over all 64 problems' tiles) and the hybrid
+    # (a run's first round in plane launches) to the bit
+    for method, mode, ref in (("stream", "fast_fp64", fp64), ("stream", "fast_fp32", mixed), ("hybrid", "fast_fp32", mixed)):
+        b.set_schedule(method, [4, 8, 16])
+        print(f"uneven batch, {method} sweep, {mode}", flush=True)
+        rc, why = b.solve(mode)
+        assert rc == 0, why
+        for q in range(len(probs)):
+            assert np.array_equal(b.read(q), ref[q]), (mode, q)
     b.release()
     order = np.argsort(sizes)
     picks = [int(order[0]), int(order[len(order) // 3]), int(order[(2 * len(order)) // 3]), int(order[-1])]

@@ -3,7 +3,10 @@ with the oracle's Relax (Forward/SphericalEikonal.cs:284-396) per problem, throu
 
 Bars: the FP64 form equals Relax to the last bit on every problem; the mixed form (first order on
 FP32 times, second order on FP64 times) stays within 1e-4 relative error; a problem's result is the
-same solved alone and in the batch, and the same after the tuner changes the launch shape.
+same solved alone and in the batch, and the same after the tuner changes the launch shape. The
+streaming sweep (one persistent launch per ordering over every problem's column tiles) updates the
+same nodes in a topological order of each ordering, so it equals the plane sweep to the bit, at every
+tile width.
 """
 import numpy as np
 import pytest
@@ -92,3 +95,79 @@ def test_declines_and_invalid(device, small_batch):
     assert e.value.status == _lib.TS_DECLINED
     with pytest.raises(ValueError):
         GridBatch(device, [probs[0].grid], [probs[0].slowness[:-1]], [seeds[0]])
+
+
+@pytest.mark.parametrize("width", [4, 8, 16])
+def test_streaming_fp64_equals_relax(device, small_batch, relax_ref, width):
+    from tomostar_amd.gridbatch import GridBatch
+
+    probs, seeds = small_batch
+    b = GridBatch(device, [p.grid for p in probs], [p.slowness for p in probs], seeds)
+    b.set_schedule("stream", [width] * 3)
+    assert b.schedule()["method"] == "stream"
+    print(f"streaming sweep, tile width {width}", flush=True)
+    rc, why = b.solve("fast_fp64")
+    assert rc == 0, why
+    for q in range(len(probs)):
+        assert np.array_equal(b.read(q), relax_ref[q]), q
+    b.release()
+
+
+def test_streaming_mixed_equals_planes_and_tuner(device, small_batch):
+    from tomostar_amd.gridbatch import GridBatch
+
+    probs, seeds = small_batch
+    b = GridBatch(device, [p.grid for p in probs], [p.slowness for p in probs], seeds)
+    b.set_schedule("planes")
+    assert b.solve("fast_fp32")[0] == 0
+    planes = [b.read(q) for q in range(len(probs))]
+    for method, widths in (("stream", [4, 8, 16]), ("stream", [16, 4, 8]), ("hybrid", [8, 8, 8])):
+        b.set_schedule(method, widths)
+        print(f"{method} mixed form, widths {widths}", flush=True)
+        rc, why = b.solve("fast_fp32")
+        assert rc == 0, why
+        for q in range(len(probs)):
+            assert np.array_equal(b.read(q), planes[q]), (method, widths, q)
+        if method == "hybrid":
+            methods = [m for _, m, _ in b.round_log()]
+            assert methods[0] == "planes" and "stream" in methods, methods
+    b.tune()
+    sch = b.schedule()
+    print(f"tuner: {sch}", flush=True)
+    assert sch["method"] in ("planes", "stream", "hybrid") and all(w in (4, 8, 16) for w in sch["tile_width"])
+    assert all(ms > 0 for ms in sch["trial_ms"].values())
+    assert b.solve("fast_fp32")[0] == 0
+    for q in range(len(probs)):
+        assert np.array_equal(b.read(q), planes[q]), q
+    b.release()
+
+
+def test_streaming_wait_limit_raises_an_error(device, small_batch, monkeypatch):
+    """A wait beyond the spin limit (forced by a limit of zero polls, read when the batch is made) is a
+    device error, not a hang; a batch made with the normal limit then solves correctly."""
+    from tomostar_amd.gridbatch import GridBatch
+
+    probs, seeds = small_batch
+    monkeypatch.setenv("TOMOSTAR_STREAM_SPIN", "0")
+    b = GridBatch(device, [p.grid for p in probs], [p.slowness for p in probs], seeds)
+    b.set_schedule("stream", [4, 4, 4])
+    rc, why = b.solve("fast_fp64")
+    assert rc != 0 and "spin limit" in why, (rc, why)
+    b.release()
+    monkeypatch.delenv("TOMOSTAR_STREAM_SPIN")
+    b = GridBatch(device, [p.grid for p in probs], [p.slowness for p in probs], seeds)
+    b.set_schedule("stream", [4, 4, 4])
+    assert b.solve("fast_fp64")[0] == 0
+    b.release()
+
+
+def test_set_schedule_rejects_bad_widths(device, small_batch):
+    from tomostar_amd import _lib
+    from tomostar_amd.gridbatch import GridBatch
+
+    probs, seeds = small_batch
+    b = GridBatch(device, [probs[0].grid], [probs[0].slowness], [seeds[0]])
+    with pytest.raises(_lib.HipError) as e:
+        b.set_schedule("stream", [4, 5, 8])
+    assert e.value.status == _lib.TS_EINVAL
+    b.release()

@@ -463,20 +463,6 @@ __device__ __forceinline__ uint64_t finish_node(const FastArgs<R> &A, const Node
     return mv;
 }
 
-// Stores and loads coherent across the device's L2 caches (write-through / miss-forcing, "sc1"): the
-// data a streaming tile exchanges with the tiles beside it within one launch. Everything else a tile
-// touches is its own, or was written by an earlier launch.
-template <typename T>
-__device__ inline void st_dev(T *p, T v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ inline T ld_dev(T *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // The update of a streaming sweep's node: as finish_node, with the new time returned (the current
 // one when no lane lowers it) for the tile's ring, the moved lanes left to the caller, and the lanes
 // that woke the node's readers returned. dev (wave-uniform): the node lies within two columns of the

@@ -349,4 +349,27 @@ __device__ inline double godunov_lean64(double s, const double first[3], const d
     return cand < inf ? r + cand : inf;
 }
 
+// Stores and loads coherent across the device's L2 caches (write-through / miss-forcing, "sc1"): the
+// data a streaming tile exchanges with the tiles beside it within one launch. Everything else a tile
+// touches is its own, or was written by an earlier launch.
+template <typename T>
+__device__ inline void st_dev(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline T ld_dev(T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// This wave's stores have completed (the fences keep the compiler from moving memory operations
+// across the wait).
+__device__ inline void wait_stores()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
 }  // namespace ts

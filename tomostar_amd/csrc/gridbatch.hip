@@ -13,6 +13,21 @@
 //
 // A tuner picks, per size class of problem (by node count), how many nodes a thread updates per
 // step (1, 2 or 4), by timing one trial round of the class alone with each choice.
+//
+// The streaming sweep (mg_stream) instead runs each ordering as ONE persistent launch over a work
+// queue of column tiles of every problem: a wave takes a tile of 64 z-columns of one problem (one
+// lane per column) and walks the ordering's diagonal planes x' + y' + z' = P through it, lane (a, b)
+// updating node z' = P - x' - y' of its column. A lane keeps its column's nodes z'-2 .. z'+2 in
+// registers and takes its x / y neighbours from the lanes beside it (shuffles; the lanes are in
+// lockstep, so a neighbour upwind already holds this plane's new time and one downwind still its
+// old one), and the neighbours beyond the tile's edge from memory, a plane ahead: old times of the
+// tiles downwind, new times of the tiles upwind once those have published the plane. Tiles are
+// queued so that each follows its upwind tiles; times, flags and progress that cross a tile edge
+// within a launch are stored and loaded device-coherent, and a wait beyond a spin limit raises an
+// error flag instead of hanging. A node is updated when the plane kernels would update it (its dirty
+// flag, or a drop of an upwind neighbour in the last two planes, which the flag cannot show yet), so
+// the results equal the plane sweep's to the bit. The tuner picks the tile width per size class and
+// the sweep (planes or streaming) for the batch.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -36,6 +51,9 @@ struct MgProblem {
     int wlon_base, wlat_base;
     double wr0, wr1;     // depth-axis spacing weights (FP64; the FP32 kernel narrows them)
     float wr0f, wr1f;
+    // streaming sweep: tile width along x' (4, 8 or 16 columns; 64 / width along y'), tiles per
+    // axis and the problem's first progress slot
+    int tile_w, ntu, ntv, slot_base;
 };
 
 struct MgSeg {
@@ -53,6 +71,16 @@ struct MgChunk {
 
 struct ts_gridbatch {
     ts_context *ctx = nullptr;
+    // streaming sweep: method (0 plane launches, 1 streaming), tile width per size class, the task
+    // list (problem | tu << 32 | tv << 48, each tile after its upwind tiles), queues and progress
+    int method = 0;
+    int tile_w[3] = {8, 8, 8};
+    dbuf<uint64_t> stasks;
+    int sntasks = 0, sslots = 0;
+    dbuf<int32_t> squeue, sprog;
+    int spin_limit = 1 << 18;
+    int publish = 4;  // planes between two publications of a tile's progress (measured: 1, 2, 4, 8 planes
+                      // take the 64-problem mixed solve 0.373, 0.342, 0.335, 0.351 s)
     int nprob = 0;
     std::vector<Grid> grids;
     std::vector<int64_t> base;            // node base per problem (+ total)
@@ -75,7 +103,10 @@ struct ts_gridbatch {
     std::vector<int> seg_blocks;          // per plane index: blocks of its launch
     std::vector<int32_t> rounds_host;
     std::vector<int> tuned;               // nodes per thread chosen per size class
+    double tune_ms[3] = {0, 0, 0};        // the tuner's trial mixed solve of the batch per method (ms)
     double seconds = 0, sweep_seconds = 0;
+    std::vector<int32_t> round_kind;      // per round of the last solve: 16 for FP64 times | method
+    std::vector<double> round_seconds;    // and its device seconds
     int last_mode = -1;
     double wake_tol[2] = {0, 0};          // MgArgs::wake_tol of the running FP32 / FP64 sweeps
     static constexpr int kMaxChunks = 4;
@@ -109,6 +140,7 @@ constexpr float kMgFp32SettleUlps = 4096;
 // batched form's does; the FP64 form wakes them on every drop, as Relax does.
 constexpr double kMgMixedWakeTol = 1e-8;
 constexpr int kMgBlock = 256;
+constexpr int kWave = 64;
 
 inline int mg_round_cap(const Grid &g)  // SphericalEikonal.cs:335
 {
@@ -295,6 +327,251 @@ __global__ void mg_widen(const float *a, double *b, int64_t n)
     if (q < n) b[q] = (double)a[q];
 }
 
+// ------------------------------------------------------------------------------ the streaming sweep
+
+template <typename R>
+struct MgStreamArgs {
+    MgArgs<R> A;
+    const uint64_t *tasks;  // problem | tu << 32 | tv << 48, each tile after its upwind tiles
+    int ntasks, slots, spin_limit, publish;
+    int32_t *queue;         // [8]: next task of ordering d
+    int32_t *prog;          // [8][slots]: planes completed + 1 per tile (INT_MAX when done)
+    int32_t *err;           // set when a wait exceeded the spin limit
+};
+
+// One tile of problem q (tile (tu, tv) of the ordering's mirrored coordinates), one wave, one lane per
+// column: the planes from the first that any column's node z' = 0 needs loaded to the last.
+template <typename R, int DIR, bool kSecond>
+__device__ __forceinline__ void mg_stream_tile(const MgStreamArgs<R> &S, const MgProblem &P, int q, int tu, int tv,
+                                               int slot, int32_t *prog, int lane, unsigned long long &evals,
+                                               bool &moved_any)
+{
+    const MgArgs<R> &A = S.A;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = nx * ny;
+    const int TU = P.tile_w, TV = kWave / TU, lg = TU == 4 ? 2 : TU == 8 ? 3 : 4;
+    const int a = lane & (TU - 1), b = lane >> lg;
+    const int u0 = tu * TU, v0 = tv * TV, xp = u0 + a, yp = v0 + b;
+    const bool real = xp < nx && yp < ny;
+    const int x = (DIR & 1) ? nx - 1 - xp : xp, y = (DIR & 2) ? ny - 1 - yp : yp;
+    const int colb = real ? y * nx + x : 0;
+    auto az = [&](int zp) { return (DIR & 4) ? nz - 1 - zp : zp; };
+    const int sx = (DIR & 1) ? -1 : 1, sy = (DIR & 2) ? -nx : nx;  // node offset of mirrored +1 along x', y'
+    R *Tp = A.T + P.node_base;
+    uint8_t *Dp = A.dirty + P.node_base;
+    const uint8_t *Sd = A.seeded + P.node_base;
+    const R *Sl = A.slow + P.node_base;
+    const R inf = (R)INFINITY;
+    // lanes whose stores tiles downwind read within the launch, and whose wakes may cross an edge
+    const bool dev = a >= TU - 2 || b >= TV - 2;
+    const bool edge = dev || a <= 1 || b <= 1;
+    const int Pb = u0 + v0 - 3, Pe = u0 + TU - 1 + v0 + TV - 1 + nz - 1;
+    int seen_u = tu > 0 ? 0 : INT_MAX, seen_v = tv > 0 ? 0 : INT_MAX;
+    // w[i]: this column's time at z' = kc - 2 + i (+inf off the grid or not loaded); pre: z' = kc + 3
+    R w[5] = {inf, inf, inf, inf, inf};
+    R pre = inf;
+    {
+        const int k3 = Pb - xp - yp + 3;
+        if (real && k3 >= 0 && k3 < nz) pre = Tp[az(k3) * nxy + colb];
+    }
+    // what plane kc and the next read beyond the window ([0]: this plane, [1]: the next, loaded two
+    // planes ahead): slowness, seed mark, dirty flag, and the neighbours beyond the tile (nearest first)
+    R sl[2] = {(R)0, (R)0}, hx[2][2] = {{inf, inf}, {inf, inf}}, hy[2][2] = {{inf, inf}, {inf, inf}};
+    uint32_t seed[2] = {1, 1}, flag[2] = {0, 0};
+    uint32_t d1 = 0, d2 = 0;  // this column dropped its node at z' = kc - 1, kc - 2 (this sweep)
+    for (int Pl = Pb; Pl <= Pe; Pl++) {
+        const int kc = Pl - xp - yp;
+        const bool in = real && kc >= 0 && kc < nz;
+        // (1) the update of node kc: dirty when its flag (loaded two planes ago) is set or an upwind
+        // neighbour dropped in the last two planes (lanes beside it, or this column)
+        uint32_t dl = flag[0] | d1 | d2;
+        {
+            const uint32_t m1 = __shfl(d1, lane - 1), m2 = __shfl(d2, lane - 2);
+            const uint32_t n1 = __shfl(d1, lane - TU), n2 = __shfl(d2, lane - 2 * TU);
+            if (a >= 1) dl |= m1;
+            if (a >= 2) dl |= m2;
+            if (b >= 1) dl |= n1;
+            if (b >= 2) dl |= n2;
+        }
+        const bool ev = in && dl != 0 && seed[0] == 0;  // evaluated: its flag is cleared
+        uint32_t dnew = 0;
+        if (__ballot(ev)) {
+            // neighbours at z' = kc: a lane upwind already holds this plane's new time at its w[1] / w[0]
+            // (one / two planes ago), a lane downwind still the old time at its w[3] / w[4]
+            R xm1 = __shfl(w[1], lane - 1), xm2 = __shfl(w[0], lane - 2);
+            R xq1 = __shfl(w[3], lane + 1), xq2 = __shfl(w[4], lane + 2);
+            R ym1 = __shfl(w[1], lane - TU), ym2 = __shfl(w[0], lane - 2 * TU);
+            R yq1 = __shfl(w[3], lane + TU), yq2 = __shfl(w[4], lane + 2 * TU);
+            xm1 = a >= 1 ? xm1 : (xp >= 1 ? hx[0][0] : inf);
+            xm2 = a >= 2 ? xm2 : (xp >= 2 ? hx[0][1] : inf);
+            xq1 = a + 1 < TU ? xq1 : (xp + 1 < nx ? hx[0][0] : inf);
+            xq2 = a + 2 < TU ? xq2 : (xp + 2 < nx ? hx[0][1] : inf);
+            ym1 = b >= 1 ? ym1 : (yp >= 1 ? hy[0][0] : inf);
+            ym2 = b >= 2 ? ym2 : (yp >= 2 ? hy[0][1] : inf);
+            yq1 = b + 1 < TV ? yq1 : (yp + 1 < ny ? hy[0][0] : inf);
+            yq2 = b + 2 < TV ? yq2 : (yp + 2 < ny ? hy[0][1] : inf);
+            // the grid's own orientation: minus side first on every axis
+            const R mn[3] = {(DIR & 1) ? xq1 : xm1, (DIR & 2) ? yq1 : ym1, (DIR & 4) ? w[3] : w[1]};
+            const R pl[3] = {(DIR & 1) ? xm1 : xq1, (DIR & 2) ? ym1 : yq1, (DIR & 4) ? w[1] : w[3]};
+            const R m2[3] = {(DIR & 1) ? xq2 : xm2, (DIR & 2) ? yq2 : ym2, (DIR & 4) ? w[4] : w[0]};
+            const R p2[3] = {(DIR & 1) ? xm2 : xq2, (DIR & 2) ? ym2 : yq2, (DIR & 4) ? w[0] : w[4]};
+            const int k = az(kc), node = k * nxy + colb;
+            const R cur = w[2];
+            if (ev) {
+                if (dev) st_dev(&Dp[node], (uint8_t)0);
+                else Dp[node] = 0;
+            }
+            const bool on = ev && sl[0] > (R)0;
+            R next = cur;
+            if (on) {
+                R first[3], t2[3];
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++) {
+                    const bool minus = mn[ax] <= pl[ax];
+                    first[ax] = minus ? mn[ax] : pl[ax];
+                    t2[ax] = (kSecond && isfinite(first[ax])) ? (minus ? m2[ax] : p2[ax]) : inf;
+                }
+                const int jl = P.wlon_base + 2 * (k * ny + y), kl = P.wlat_base + 2 * k;
+                if constexpr (sizeof(R) == 4) {
+                    const float wa[3][2] = {{A.wlon[jl], A.wlon[jl + 1]}, {A.wlat[kl], A.wlat[kl + 1]}, {P.wr0f, P.wr1f}};
+                    next = godunov_offset(sl[0], first, t2, wa);
+                } else {
+                    const double wa[3][2] = {{A.wlon[jl], A.wlon[jl + 1]}, {A.wlat[kl], A.wlat[kl + 1]}, {P.wr0, P.wr1}};
+                    next = godunov_abs_w<double>(sl[0], first, t2, wa);
+                }
+            }
+            bool moved = false, dropped = false;
+            if (on && next < cur) {
+                if (dev) st_dev(&Tp[node], next);
+                else Tp[node] = next;
+                w[2] = next;
+                if constexpr (sizeof(R) == 4)
+                    moved = !isfinite(cur) || (cur - next) > fmaxf((float)kMgSettleTol, kMgFp32SettleUlps * 1.1920929e-7f * cur);
+                else
+                    moved = !isfinite(cur) || (double)(cur - next) > kMgSettleTol;
+                dropped = !isfinite(cur) || (double)(cur - next) > A.wake_tol;
+            }
+            if (dropped) {
+                // wake the twelve readers (one and two steps on each axis) inside the grid
+                const int pos[3] = {x, y, k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++)
+#pragma unroll
+                    for (int dd = -2; dd <= 2; dd++) {
+                        if (dd == 0 || pos[ax] + dd < 0 || pos[ax] + dd >= n[ax]) continue;
+                        uint8_t *f = &Dp[node + dd * st[ax]];
+                        if (edge) st_dev(f, (uint8_t)1);
+                        else *f = 1;
+                    }
+            }
+            dnew = dropped;
+            moved_any |= __ballot(moved) != 0;
+            evals += (unsigned long long)__popcll(__ballot(on));
+        }
+        // (2) the window moves one node down the column, and the next plane's reads become this one's
+        w[0] = w[1];
+        w[1] = w[2];
+        w[2] = w[3];
+        w[3] = w[4];
+        w[4] = pre;
+        d2 = d1;
+        d1 = dnew;
+        sl[0] = sl[1];
+        seed[0] = seed[1];
+        flag[0] = flag[1];
+        hx[0][0] = hx[1][0];
+        hx[0][1] = hx[1][1];
+        hy[0][0] = hy[1][0];
+        hy[0][1] = hy[1][1];
+        if ((Pl - Pb + 1) % S.publish == 0 || Pl == Pe) {
+            // the stores other tiles read are complete before the plane is published
+            wait_stores();
+            if (lane == 0) __hip_atomic_store(&prog[slot], Pl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // (3) loads for later planes: this column's node kc + 4 (the window's new bottom a plane later),
+        // and for node kc + 2 (two planes later) its slowness, seed mark, neighbours beyond the tile
+        // and dirty flag
+        const int kn = kc + 2, k4 = kc + 4;
+        pre = (real && k4 >= 0 && k4 < nz) ? Tp[az(k4) * nxy + colb] : inf;
+        const bool inn = real && kn >= 0 && kn < nz;
+        const int nn = az(inn ? kn : 0) * nxy + colb;
+        sl[1] = inn ? Sl[nn] : (R)0;
+        seed[1] = inn ? Sd[nn] : 1;
+        if (inn && a >= TU - 2) {  // downwind along x': old times
+            hx[1][0] = (a == TU - 1 && xp + 1 < nx) ? Tp[nn + sx] : inf;
+            if (kSecond) hx[1][1] = xp + 2 < nx ? Tp[nn + 2 * sx] : inf;
+        }
+        if (inn && b >= TV - 2) {
+            hy[1][0] = (b == TV - 1 && yp + 1 < ny) ? Tp[nn + sy] : inf;
+            if (kSecond) hy[1][1] = yp + 2 < ny ? Tp[nn + 2 * sy] : inf;
+        }
+        // the upwind tiles must have completed plane Pl + 1 (their nodes beside plane Pl + 2's, and
+        // the flags they raised)
+        auto wait_for = [&](int &seen, int at) {
+            if (seen > Pl + 1) return;
+            int spins = 0;
+            for (;;) {
+                int v = __hip_atomic_load(&prog[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                v = __builtin_amdgcn_readfirstlane(v);
+                if (v > Pl + 1) { seen = v; break; }
+                if (++spins > S.spin_limit) {
+                    if (lane == 0) atomicExch(S.err, 1);
+                    seen = INT_MAX;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        };
+        if (tu > 0) wait_for(seen_u, slot - 1);
+        if (tv > 0) wait_for(seen_v, slot - P.ntu);
+        if (inn && a <= 1) {  // upwind along x': new times
+            hx[1][0] = (a == 0 && xp >= 1) ? ld_dev(&Tp[nn - sx]) : inf;
+            if (kSecond) hx[1][1] = xp >= 2 ? ld_dev(&Tp[nn - 2 * sx]) : inf;
+        }
+        if (inn && b <= 1) {
+            hy[1][0] = (b == 0 && yp >= 1) ? ld_dev(&Tp[nn - sy]) : inf;
+            if (kSecond) hy[1][1] = yp >= 2 ? ld_dev(&Tp[nn - 2 * sy]) : inf;
+        }
+        flag[1] = !inn ? 0u : (a <= 1 || b <= 1) ? (uint32_t)ld_dev(&Dp[nn]) : (uint32_t)Dp[nn];
+    }
+}
+
+// One sweep ordering (DIR) of every problem's tiles as a persistent launch: each wave takes tiles from
+// the queue until it is empty (the waves of a workgroup are independent).
+template <typename R, int DIR>
+__global__ __launch_bounds__(256) void mg_stream(MgStreamArgs<R> S)
+{
+    const int lane = threadIdx.x & 63;
+    int32_t *prog = S.prog + (size_t)DIR * S.slots;
+    unsigned long long evals = 0;
+    for (;;) {
+        int task = 0;
+        if (lane == 0) task = atomicAdd(&S.queue[DIR], 1);
+        task = __builtin_amdgcn_readfirstlane(task);
+        if (task >= S.ntasks) break;
+        const uint64_t pk = S.tasks[task];
+        const int q = (int)(pk & 0xffffffffu), tu = (int)((pk >> 32) & 0xffff), tv = (int)(pk >> 48);
+        const MgProblem P = S.A.prob[q];
+        const int slot = P.slot_base + tv * P.ntu + tu;
+        const int ph = S.A.phase[q];
+        bool moved = false;
+        if (ph == 1) mg_stream_tile<R, DIR, true>(S, P, q, tu, tv, slot, prog, lane, evals, moved);
+        else if (ph == 0) mg_stream_tile<R, DIR, false>(S, P, q, tu, tv, slot, prog, lane, evals, moved);
+        if (moved && lane == 0) atomicOr(&S.A.changed[q], 1);
+        // done: the tiles downwind wait no more
+        wait_stores();
+        if (lane == 0) __hip_atomic_store(&prog[slot], INT_MAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0 && evals) atomicAdd(&S.A.evals[sizeof(R) == 8], evals);
+}
+
+__global__ void mg_stream_reset(int32_t *queue, int32_t *prog, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < 8) queue[q] = 0;
+    if (q < n) prog[q] = 0;
+}
+
 unsigned mg_blocks(int64_t n, int bs) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
 
 int size_class(int64_t nodes) { return nodes < 200000 ? 0 : nodes < 800000 ? 1 : 2; }
@@ -400,16 +677,101 @@ void mg_planes(ts_gridbatch &b, hipStream_t st, const std::vector<int> &seg_off,
         }
 }
 
+// The streaming sweep's tiles: per problem the tile width of its size class, its tiles per axis and
+// progress slots; the task list by tile diagonal, then problem, then tile, so that every tile follows
+// its upwind tiles (tu - 1 and tv - 1 of the same problem).
+void build_stream_plan(ts_gridbatch &b)
+{
+    int slots = 0, maxd = 0;
+    for (int q = 0; q < b.nprob; q++) {
+        MgProblem &P = b.prob[q];
+        const int tw = b.tile_w[size_class(b.base[q + 1] - b.base[q])];
+        P.tile_w = tw;
+        P.ntu = (P.nx + tw - 1) / tw;
+        P.ntv = (P.ny + kWave / tw - 1) / (kWave / tw);
+        P.slot_base = slots;
+        slots += P.ntu * P.ntv;
+        maxd = std::max(maxd, P.ntu - 1 + P.ntv - 1);
+    }
+    std::vector<uint64_t> list;
+    list.reserve(slots);
+    for (int d = 0; d <= maxd; d++)
+        for (int q = 0; q < b.nprob; q++) {
+            const MgProblem &P = b.prob[q];
+            for (int tu = std::max(0, d - (P.ntv - 1)); tu <= std::min(P.ntu - 1, d); tu++)
+                list.push_back((uint64_t)q | (uint64_t)tu << 32 | (uint64_t)(d - tu) << 48);
+        }
+    if ((int64_t)list.size() != slots) fail(TS_EDEVICE, "streaming plan covers %zu of %d tiles", list.size(), slots);
+    b.stasks.alloc(list.size());
+    TS_HIP(hipMemcpy(b.stasks.p, list.data(), list.size() * 8, hipMemcpyHostToDevice));
+    b.sntasks = (int)list.size();
+    b.sslots = slots;
+    b.sprog.alloc((size_t)8 * slots);
+    b.squeue.alloc(8);
+    TS_HIP(hipMemcpy(b.prob_dev.p, b.prob.data(), b.nprob * sizeof(MgProblem), hipMemcpyHostToDevice));
+}
+
+// Workgroups (of four independent waves) of a streaming launch: as many as stay resident.
+template <typename R, int DIR>
+int mg_stream_grid(const ts_gridbatch &b)
+{
+    static const int per_cu = [] {
+        int n = 0;
+        TS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mg_stream<R, DIR>, 256, 0));
+        return std::max(1, n);
+    }();
+    const int needed = (b.sntasks + 3) / 4;
+    return std::max(1, std::min(needed, per_cu * b.ctx->cu_count));
+}
+
+template <typename R, int DIR>
+void mg_stream_dir(ts_gridbatch &b, const MgStreamArgs<R> &S, hipStream_t st)
+{
+    mg_stream<R, DIR><<<mg_stream_grid<R, DIR>(b), 256, 0, st>>>(S);
+}
+
+template <typename R>
+void mg_stream_planes(ts_gridbatch &b, hipStream_t st)
+{
+    MgStreamArgs<R> S;
+    S.A = mg_args<R>(b);
+    S.tasks = b.stasks.p;
+    S.ntasks = b.sntasks;
+    S.slots = b.sslots;
+    S.spin_limit = b.spin_limit;
+    S.publish = b.publish;
+    S.queue = b.squeue.p;
+    S.prog = b.sprog.p;
+    S.err = b.status.p + 2;
+    const int n = 8 * b.sslots;
+    mg_stream_reset<<<mg_blocks(std::max(n, 8), 256), 256, 0, st>>>(b.squeue.p, b.sprog.p, n);
+    mg_stream_dir<R, 0>(b, S, st);
+    mg_stream_dir<R, 1>(b, S, st);
+    mg_stream_dir<R, 2>(b, S, st);
+    mg_stream_dir<R, 3>(b, S, st);
+    mg_stream_dir<R, 4>(b, S, st);
+    mg_stream_dir<R, 5>(b, S, st);
+    mg_stream_dir<R, 6>(b, S, st);
+    mg_stream_dir<R, 7>(b, S, st);
+    TS_HIP(hipGetLastError());
+}
+
 void mg_end(ts_gridbatch &b, hipStream_t st, int hold)
 {
     mg_round_end<<<mg_blocks(b.nprob, 64), 64, 0, st>>>(b.nprob, b.phase.p, b.changed.p, b.rounds.p, b.cap.p,
                                                        b.status.p, b.reset.p, hold);
 }
 
+// The sweep round `round` of a run uses: method 0 plane launches, 1 streaming, 2 (hybrid) plane
+// launches for a run's first round (every node dirty: the plane launches keep more updates in flight)
+// and streaming after it (few dirty nodes: one launch per ordering instead of one per plane step).
+inline int round_method(const ts_gridbatch &b, int round) { return b.method == 2 ? (round == 0 ? 0 : 1) : b.method; }
+
 template <typename R>
-void mg_round(ts_gridbatch &b, hipStream_t st, int hold)
+void mg_round(ts_gridbatch &b, hipStream_t st, int hold, int method)
 {
-    mg_planes<R>(b, st, b.seg_off, b.seg_blocks, b.segs.p);
+    if (method == 1) mg_stream_planes<R>(b, st);
+    else mg_planes<R>(b, st, b.seg_off, b.seg_blocks, b.segs.p);
     mg_end(b, st, hold);
 }
 
@@ -495,17 +857,21 @@ void mg_rounds(ts_gridbatch &b, const volatile int32_t *cancel, int hold, int ma
     const int64_t total = b.base[b.nprob];
     for (int round = 0; round < max_rounds; round++) {
         if (cancel && *cancel) fail(TS_CANCELLED, "cancelled");
-        TS_HIP(hipMemsetAsync(b.status.p, 0, 8, st));
+        TS_HIP(hipMemsetAsync(b.status.p, 0, 12, st));
         TS_HIP(hipEventRecord(e0, st));
-        if (b.nchunks > 1) mg_round_chunks<R>(b, st, hold);
-        else mg_round<R>(b, st, hold);
+        const int method = round_method(b, round);
+        if (b.nchunks > 1 && method == 0) mg_round_chunks<R>(b, st, hold);
+        else mg_round<R>(b, st, hold, method);
         TS_HIP(hipEventRecord(e1, st));
         if (!hold) mg_reset_dirty<<<mg_blocks(total, 256), 256, 0, st>>>(b.dirty.p, d_base, b.nprob, b.reset.p, total);
-        TS_HIP(hipMemcpyAsync(h_status, b.status.p, 8, hipMemcpyDeviceToHost, st));
+        TS_HIP(hipMemcpyAsync(h_status, b.status.p, 12, hipMemcpyDeviceToHost, st));
         TS_HIP(hipStreamSynchronize(st));
         float ms = 0;
         TS_HIP(hipEventElapsedTime(&ms, e0, e1));
         b.sweep_seconds += ms * 1e-3;
+        b.round_kind.push_back((sizeof(R) == 8 ? 16 : 0) | method);
+        b.round_seconds.push_back(ms * 1e-3);
+        if (h_status[2] > 0) fail(TS_EDEVICE, "streaming sweep: a tile waited for its upwind tiles beyond the spin limit");
         if (h_status[1] > 0) fail(TS_DECLINED, "%d problem(s) not settled within their round cap", h_status[1]);
         if (h_status[0] == 0) return;
     }
@@ -520,11 +886,13 @@ void mg_solve(ts_gridbatch &b, int mode, const volatile int32_t *cancel)
     dbuf<int64_t> d_base(b.nprob);
     TS_HIP(hipMemcpy(d_base.p, b.base.data(), b.nprob * 8, hipMemcpyHostToDevice));
     int32_t *h_status = nullptr;
-    TS_HIP(hipHostMalloc((void **)&h_status, 8, hipHostMallocDefault));
+    TS_HIP(hipHostMalloc((void **)&h_status, 12, hipHostMallocDefault));
     struct FreeH { int32_t *p; ~FreeH() { if (p) (void)hipHostFree(p); } } fh{h_status};
     int max_rounds = 0;
     for (const Grid &g : b.grids) max_rounds = std::max(max_rounds, 2 * mg_round_cap(g) + 2);
     b.sweep_seconds = 0;
+    b.round_kind.clear();
+    b.round_seconds.clear();
     TS_HIP(hipMemsetAsync(b.evals.p, 0, 2 * sizeof(unsigned long long), st));
     b.wake_tol[0] = 0.0;
     b.wake_tol[1] = mixed ? kMgMixedWakeTol : 0.0;
@@ -578,7 +946,7 @@ void mg_tune(ts_gridbatch &b)
             TS_HIP(hipEventCreate(&e0));
             TS_HIP(hipEventCreate(&e1));
             TS_HIP(hipEventRecord(e0, st));
-            mg_round<float>(b, st, 1);
+            mg_round<float>(b, st, 1, 0);
             TS_HIP(hipEventRecord(e1, st));
             TS_HIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -591,6 +959,61 @@ void mg_tune(ts_gridbatch &b)
     }
     TS_HIP(hipMemcpy(b.prob_dev.p, b.prob.data(), b.nprob * sizeof(MgProblem), hipMemcpyHostToDevice));
     build_segments(b, {});
+    // the streaming sweep: per class the tile width whose trial round of the class alone (the other
+    // problems held done) is fastest, then the batch's sweep by one trial round of each
+    auto trial = [&](int method, const std::vector<int> &which) {
+        mg_start(b, true, st);
+        if (!which.empty()) {
+            std::vector<int32_t> ph(b.nprob, 2);
+            for (int q : which) ph[q] = 0;
+            TS_HIP(hipMemcpy(b.phase.p, ph.data(), b.nprob * 4, hipMemcpyHostToDevice));
+        }
+        hipEvent_t e0, e1;
+        TS_HIP(hipEventCreate(&e0));
+        TS_HIP(hipEventCreate(&e1));
+        TS_HIP(hipMemsetAsync(b.status.p, 0, 12, st));
+        TS_HIP(hipEventRecord(e0, st));
+        mg_round<float>(b, st, 1, method);
+        TS_HIP(hipEventRecord(e1, st));
+        TS_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        TS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        int32_t err = 0;
+        TS_HIP(hipMemcpy(&err, b.status.p + 2, 4, hipMemcpyDeviceToHost));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (err) fail(TS_EDEVICE, "streaming sweep: a tile waited for its upwind tiles beyond the spin limit");
+        return (double)ms;
+    };
+    const int widths[3] = {4, 8, 16};
+    for (int c = 0; c < 3; c++) {
+        std::vector<int> which;
+        for (int q = 0; q < b.nprob; q++)
+            if (size_class(b.base[q + 1] - b.base[q]) == c) which.push_back(q);
+        if (which.empty()) continue;
+        double best = 1e300;
+        int pick = b.tile_w[c];
+        for (int tw : widths) {
+            b.tile_w[c] = tw;
+            build_stream_plan(b);
+            const double ms = trial(1, which);
+            if (ms < best) { best = ms; pick = tw; }
+        }
+        b.tile_w[c] = pick;
+    }
+    build_stream_plan(b);
+    int pick_method = b.method;
+    // the method: a whole mixed solve with each (a first round alone misleads: streaming is slower
+    // while every node is dirty and faster once few are)
+    double best = 1e300;
+    for (int m = 0; m < 3; m++) {
+        b.method = m;
+        const auto t0 = std::chrono::steady_clock::now();
+        mg_solve(b, TS_EIK_FAST_FP32, nullptr);
+        b.tune_ms[m] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (b.tune_ms[m] < best) { best = b.tune_ms[m]; pick_method = m; }
+    }
+    b.method = pick_method;
 }
 
 }  // namespace
@@ -711,13 +1134,16 @@ int ts_gridbatch_create(ts_context *ctx, int nproblems, const ts_grid *grids, co
         b.seeded.alloc(total);
         b.prob_dev.alloc(nproblems);
         b.phase.alloc(nproblems); b.changed.alloc(nproblems); b.rounds.alloc(2 * nproblems);
-        b.reset.alloc(nproblems); b.cap.alloc(nproblems); b.status.alloc(2);
+        b.reset.alloc(nproblems); b.cap.alloc(nproblems); b.status.alloc(3);
         b.evals.alloc(2);
         std::vector<int32_t> caps(nproblems);
         for (int q = 0; q < nproblems; q++) caps[q] = mg_round_cap(b.grids[q]);
         TS_HIP(hipMemcpy(b.cap.p, caps.data(), nproblems * 4, hipMemcpyHostToDevice));
         TS_HIP(hipMemcpy(b.prob_dev.p, b.prob.data(), nproblems * sizeof(MgProblem), hipMemcpyHostToDevice));
         build_segments(b, {});
+        if (const char *e = getenv("TOMOSTAR_STREAM_SPIN")) b.spin_limit = std::max(0, atoi(e));
+        if (const char *e = getenv("TOMOSTAR_STREAM_PUBLISH")) b.publish = std::clamp(atoi(e), 1, 64);
+        build_stream_plan(b);
         b.rounds_host.assign(2 * (size_t)nproblems, 0);
         *out = B.release();
         return TS_OK;
@@ -732,6 +1158,51 @@ int ts_gridbatch_tune(ts_gridbatch *b, int32_t choice[3])
         TS_HIP(hipSetDevice(b->ctx->device));
         mg_tune(*b);
         if (choice) for (int c = 0; c < 3; c++) choice[c] = b->tuned[c];
+        return TS_OK;
+    });
+}
+
+int ts_gridbatch_schedule(ts_gridbatch *b, int32_t *method, int32_t tile_w[3], int32_t nodes_per_thread[3],
+                          double trial_ms[3])
+{
+    if (!b) return TS_EINVAL;
+    std::lock_guard<std::mutex> lk(b->gate);
+    if (method) *method = b->method;
+    for (int c = 0; c < 3; c++) {
+        if (tile_w) tile_w[c] = b->tile_w[c];
+        if (nodes_per_thread) nodes_per_thread[c] = b->tuned.empty() ? 1 : b->tuned[c];
+    }
+    if (trial_ms)
+        for (int m = 0; m < 3; m++) trial_ms[m] = b->tune_ms[m];
+    return TS_OK;
+}
+
+int ts_gridbatch_round_log(ts_gridbatch *b, int max, int32_t *kind, double *seconds)
+{
+    if (!b || max < 0 || (max > 0 && (!kind || !seconds))) return TS_EINVAL;
+    std::lock_guard<std::mutex> lk(b->gate);
+    const int n = (int)std::min<size_t>((size_t)max, b->round_kind.size());
+    for (int q = 0; q < n; q++) {
+        kind[q] = b->round_kind[q];
+        seconds[q] = b->round_seconds[q];
+    }
+    return (int)b->round_kind.size();
+}
+
+int ts_gridbatch_set_schedule(ts_gridbatch *b, int method, const int32_t tile_w[3])
+{
+    if (!b || method < 0 || method > 2) return TS_EINVAL;
+    if (tile_w)
+        for (int c = 0; c < 3; c++)
+            if (tile_w[c] != 4 && tile_w[c] != 8 && tile_w[c] != 16) return TS_EINVAL;
+    std::lock_guard<std::mutex> lk(b->gate);
+    return guarded(nullptr, 0, [&]() -> int {
+        TS_HIP(hipSetDevice(b->ctx->device));
+        b->method = method;
+        if (tile_w) {
+            for (int c = 0; c < 3; c++) b->tile_w[c] = tile_w[c];
+            build_stream_plan(*b);
+        }
         return TS_OK;
     });
 }

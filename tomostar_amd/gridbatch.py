@@ -14,6 +14,7 @@ from .eikonal import Device
 from .grid import GridDefinition
 
 MODES = {"fast_fp32": TS_EIK_FAST_FP32, "fast_fp64": TS_EIK_FAST_FP64}
+METHODS = ["planes", "stream", "hybrid"]  # ts_gridbatch_schedule's method codes
 
 
 def _declare():
@@ -33,6 +34,12 @@ def _declare():
     L.ts_gridbatch_stats.argtypes = [vp, vp, vp, vp]
     L.ts_gridbatch_updates.restype = i
     L.ts_gridbatch_updates.argtypes = [vp, vp, vp]
+    L.ts_gridbatch_schedule.restype = i
+    L.ts_gridbatch_schedule.argtypes = [vp, vp, vp, vp, vp]
+    L.ts_gridbatch_set_schedule.restype = i
+    L.ts_gridbatch_set_schedule.argtypes = [vp, i, vp]
+    L.ts_gridbatch_round_log.restype = i
+    L.ts_gridbatch_round_log.argtypes = [vp, i, vp, vp]
     L.ts_gridbatch_release.restype = None
     L.ts_gridbatch_release.argtypes = [vp]
     L._gridbatch_declared = True
@@ -70,10 +77,28 @@ class GridBatch:
         self.n = n
 
     def tune(self):
-        """The tuner's pick of nodes per thread for problems of < 2e5, < 8e5 and >= 8e5 nodes."""
+        """Runs the tuner; returns its pick of nodes per thread for problems of < 2e5, < 8e5 and >= 8e5 nodes
+        (the streaming tile widths and the method it picked: schedule())."""
         out = np.zeros(3, np.int32)
         check(_declare().ts_gridbatch_tune(self._h, ptr(out)))
         return [int(x) for x in out]
+
+    def schedule(self) -> dict:
+        """The sweep in use: method ("planes": one launch per plane step; "stream": one persistent launch per
+        ordering over every problem's column tiles), the streaming tile width and the plane method's nodes
+        per thread per size class, and the tuner's trial round of the batch with each method (ms)."""
+        m = C.c_int32()
+        tw, npt, ms = np.zeros(3, np.int32), np.zeros(3, np.int32), np.zeros(3)
+        check(_declare().ts_gridbatch_schedule(self._h, C.byref(m), ptr(tw), ptr(npt), ptr(ms)))
+        return {"method": METHODS[m.value], "tile_width": [int(x) for x in tw],
+                "nodes_per_thread": [int(x) for x in npt],
+                "trial_ms": {name: float(ms[q]) for q, name in enumerate(METHODS)}}
+
+    def set_schedule(self, method: str, tile_width: Sequence[int] | None = None):
+        """Forces the sweep ("planes", "stream", or "hybrid": a run's first round in plane launches, the rest
+        streaming) and, when given, the tile width (4, 8 or 16) per size class."""
+        tw = None if tile_width is None else np.ascontiguousarray(tile_width, np.int32)
+        check(_declare().ts_gridbatch_set_schedule(self._h, METHODS.index(method), ptr(tw)))
 
     def solve(self, mode: str = "fast_fp32", cancel: C.c_int32 | None = None):
         """(status, reason); rounds per problem in self.rounds (first order, second order)."""
@@ -83,6 +108,16 @@ class GridBatch:
                                            ptr(rounds), r, len(r))
         self.rounds = rounds.reshape(self.n, 2)
         return rc, r.value.decode(errors="replace")
+
+    def round_log(self):
+        """[(width, method, seconds)] per round of the last solve: width 32 or 64 (the times swept), the method
+        that swept the round ("planes" or "stream")."""
+        L = _declare()
+        n = L.ts_gridbatch_round_log(self._h, 0, None, None)
+        check(min(n, 0))
+        kind, secs = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1))
+        L.ts_gridbatch_round_log(self._h, n, ptr(kind), ptr(secs))
+        return [(64 if k & 16 else 32, METHODS[k & 15], float(x)) for k, x in zip(kind[:n], secs[:n])]
 
     def read(self, q: int) -> np.ndarray:
         out = np.zeros(self.grids[q].count)
