@@ -121,13 +121,17 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<R> &S, R *ring, uin
             const int kc = P - (u0 + a) - (v0 + b);
             if (!real[c] || kc < 0 || kc >= nz) continue;
             const int sk = kc % RG;
+            // (a drop two nodes away wakes a node only while a field of the group is in its
+            // second-order phase, as in the plane kernels)
             bool dirty = cs[c].flag != 0;
             if (kc >= 1) dirty |= drop[col * RG + (kc - 1) % RG] != 0;
-            if (kc >= 2) dirty |= drop[col * RG + (kc - 2) % RG] != 0;
             if (a >= 1) dirty |= drop[(col - 1) * RG + sk] != 0;
-            if (a >= 2) dirty |= drop[(col - 2) * RG + sk] != 0;
             if (b >= 1) dirty |= drop[(col - TU) * RG + sk] != 0;
-            if (b >= 2) dirty |= drop[(col - 2 * TU) * RG + sk] != 0;
+            if constexpr (kSecond) {
+                if (kc >= 2) dirty |= drop[col * RG + (kc - 2) % RG] != 0;
+                if (a >= 2) dirty |= drop[(col - 2) * RG + sk] != 0;
+                if (b >= 2) dirty |= drop[(col - 2 * TU) * RG + sk] != 0;
+            }
             if (!dirty) continue;
             auto ent = [&](int cl, int k) -> R { return ring[((cl * RG) + k % RG) * kLanes + lane]; };
             const int xp = u0 + a, yp = v0 + b;
