@@ -284,9 +284,20 @@ struct NodeLoad {
     R wlon0, wlon1, wlat0, wlat1;  // spacing weights of the x and y axes (first, second order)
 };
 
+// The lane whose field's times a lane loads: its own while its field relaxes this round, else the
+// first such lane of the wave. A lane whose field is settled, done or unused never stores what it
+// loads (node_update returns its current time with on false), so its loads only cost HBM rows: sent
+// to a row the wave fetches anyway, they add no cache lines (a group of 16 fields fetches 16 fields'
+// lines of each row rather than 64). act != 0 (the kernels return on an idle group).
+__device__ __forceinline__ int load_lane(uint64_t act, int lane)
+{
+    return ((act >> lane) & 1) ? lane : __builtin_ctzll(act);
+}
+
 // The loads of the update of plane node pk for the 64 fields of group g (one wavefront, one lane per
-// field). No node of a plane reads another node of the same plane (neighbours lie one or two planes
-// away), so the loads of several nodes of a plane may be issued before any of them is updated.
+// field; lane: the field whose times this lane loads, load_lane). No node of a plane reads another
+// node of the same plane (neighbours lie one or two planes away), so the loads of several nodes of a
+// plane may be issued before any of them is updated.
 template <typename R, bool kOff32, int kFlag, bool kSecond, bool kBranchLoads = (sizeof(R) == 8)>
 __device__ __forceinline__ void load_node(const FastArgs<R> &A, int dir, uint32_t pk, int g, int lane, int mdl,
                                  NodeLoad<R> &L)
@@ -539,11 +550,12 @@ __device__ __forceinline__ void plane_nodes(const FastArgs<R> &A, int dir, int o
                                             uint64_t ph, int lane, int a, int stride)
 {
     const int wake_off = wake_offset(lane, A.nx, A.nx * A.ny);
-    const int mdl = A.model[g * kLanes + lane];
+    const int ll = load_lane(act, lane);
+    const int mdl = A.model[g * kLanes + ll];
     uint64_t mv = 0;
     for (; a < cnt; a += stride) {
         NodeLoad<R> L;
-        load_node<R, kOff32, kFlag, kSecond, false>(A, dir, A.planes[off + a], g, lane, mdl, L);
+        load_node<R, kOff32, kFlag, kSecond, false>(A, dir, A.planes[off + a], g, ll, mdl, L);
         mv |= finish_node<R, kCount, kLean, kSecond>(A, L, g, act, ph, lane, blockIdx.x, wake_off);
     }
     mark_changed(&A.changed[g], mv, lane);
@@ -584,7 +596,8 @@ __device__ __forceinline__ void sparse_nodes(const FastArgs<R> &A, int dir, int 
                                              int lane, uint32_t pk, uint64_t todo)
 {
     const int wake_off = wake_offset(lane, A.nx, A.nx * A.ny);
-    const int mdl = A.model[g * kLanes + lane];
+    const int ll = load_lane(act, lane);
+    const int mdl = A.model[g * kLanes + ll];
     uint64_t mv = 0;
     while (todo) {
         uint32_t p[K];
@@ -599,7 +612,7 @@ __device__ __forceinline__ void sparse_nodes(const FastArgs<R> &A, int dir, int 
         }
         NodeLoad<R> L[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) load_node<R, kOff32, kFlagKnown, kSecond, (K == 1)>(A, dir, p[q], g, lane, mdl, L[q]);
+        for (int q = 0; q < K; q++) load_node<R, kOff32, kFlagKnown, kSecond, (K == 1)>(A, dir, p[q], g, ll, mdl, L[q]);
 #pragma unroll
         for (int q = 0; q < K; q++)
             if (valid[q]) mv |= finish_node<R, kCount, kLean, kSecond>(A, L[q], g, act, ph, lane, blockIdx.x, wake_off);

@@ -76,7 +76,8 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<R> &S, R *ring, uin
     const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
     const size_t gn = (size_t)g * A.N;
     R *Tg = A.T + gn * kLanes;
-    const int mdl = A.model[g * kLanes + lane];
+    const int ll = load_lane(act, lane);  // loads of an idle lane go to a row the wave fetches anyway
+    const int mdl = A.model[g * kLanes + ll];
     const R inf = (R)INFINITY;
     const int u0 = tu * TU, v0 = tv * TV;
     int *prog = S.sync.prog + (size_t)d * S.slots;
@@ -100,7 +101,7 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<R> &S, R *ring, uin
     auto ay = [&](int yp) { return (DIR & 2) ? ny - 1 - yp : yp; };
     auto az = [&](int zp) { return (DIR & 4) ? nz - 1 - zp : zp; };
     auto node_at = [&](int xp, int yp, int zp) { return (az(zp) * ny + ay(yp)) * nx + ax(xp); };
-    auto tval = [&](int node) -> R { return Tg[(size_t)node * kLanes + lane]; };
+    auto tval = [&](int node) -> R { return Tg[(size_t)node * kLanes + ll]; };
 
     const int Pb = u0 + v0 - 3, Pe = u0 + TU - 1 + v0 + TV - 1 + nz - 1;
     // prologue: the entry the first plane fills (z' = kc(Pb) + 3)
@@ -251,7 +252,7 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<R> &S, R *ring, uin
             if (!real[c] || kn < 0 || kn >= nz) continue;
             // times the upwind tiles wrote within this launch, and flags they may have raised: read
             // device-coherent
-            auto tdev = [&](int node) -> R { return ld_dev(&Tg[(size_t)node * kLanes + lane]); };
+            auto tdev = [&](int node) -> R { return ld_dev(&Tg[(size_t)node * kLanes + ll]); };
             if (a <= 1) {
                 if (a == 0) cs[c].hu[0] = xp >= 1 ? tdev(node_at(xp - 1, yp, kn)) : inf;
                 if (kSecond) cs[c].hu[1] = xp >= 2 ? tdev(node_at(xp - 2, yp, kn)) : inf;
