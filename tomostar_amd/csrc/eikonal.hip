@@ -46,6 +46,13 @@ constexpr int kLanes = 64;
 // (3, 1.029 s, 4.571e-6), 16384 (3, 1.031 s, 4.571e-6), 65536 (3, 1.014 s, 2.2e-5) - the second-order
 // phase takes 6 rounds in each case; from 65536 ulps on its result moves.
 constexpr float kFp32SettleUlps = 4096;  // TOMOSTAR_FP32_SETTLE_ULPS overrides
+// The mixed form's first-order phase wakes a node's readers only when it dropped by more than this many
+// FP32 ulps of its time (1.9e-6 of it): smaller drops of a phase that only seeds the second-order phase
+// are left to that phase, which starts with every node dirty. Measured on 400 fields of
+// 201 x 201 x 101 (solve seconds, first-order phase seconds, largest and RMS relative distance to the
+// FP64 form over 35 fields): 0 ulps (0.799, 0.226, 9.5e-6, 6.3e-9), 16 (0.755, 0.183, 1.6e-5, 1.2e-8),
+// 32 (0.749, 0.176, 1.8e-5, 2.2e-8), 64 (0.741, 0.168, 2.4e-5, 4.4e-8), 128 (0.735, 0.163, 6.5e-5, 9.7e-8).
+constexpr float kFp32WakeUlps = 16;  // TOMOSTAR_FP32_WAKE_ULPS overrides
 
 inline int sweep_round_cap(const Grid &g)  // SphericalEikonal.cs:335
 {
@@ -230,6 +237,7 @@ struct FastArgs {
     const uint32_t *planes;       // nodes of the plane family, packed i' | j' << 11 | k' << 22
     double wake_tol;              // a drop wakes the node's readers when larger than this (0: any drop)
     float settle_ulps;            // FP32 times: a node moved when it dropped by more than this many ulps
+    float wake_ulps;              // FP32 times: a drop wakes the node's readers when larger than this many ulps
     int nx, ny, nz, N;
 };
 
@@ -414,7 +422,10 @@ __device__ __forceinline__ R node_update(const FastArgs<R> &A, const NodeLoad<R>
 template <typename R>
 __device__ __forceinline__ void drop_tests(const FastArgs<R> &A, R cur, R next, bool &dropped, bool &moved)
 {
-    dropped = !isfinite(cur) || (double)(cur - next) > A.wake_tol;
+    if constexpr (sizeof(R) == 4)
+        dropped = !isfinite(cur) || (cur - next) > fmaxf((float)A.wake_tol, A.wake_ulps * 1.1920929e-7f * cur);
+    else
+        dropped = !isfinite(cur) || (double)(cur - next) > A.wake_tol;
     if constexpr (sizeof(R) == 4)
         moved = !isfinite(cur) || (cur - next) > fmaxf((float)kSettleTol, A.settle_ulps * 1.1920929e-7f * cur);
     else
@@ -824,12 +835,14 @@ struct ts_tableset {
     double wake_tol[2] = {0, 0};          // FastArgs::wake_tol of the running solve's FP32 / FP64 sweeps
     bool lean[2] = {false, false};        // the running solve's FP32 / FP64 sweeps use the mixed form's arithmetic
     float settle_ulps = 4096;             // the running solve's FP32 settle threshold (ulps of the time)
+    float wake_ulps = 0;                  // and its FP32 wake threshold (ulps of the time)
     // launch-shape knobs, read once when the table set is created (graphs captured with them stay valid)
     int knob_group_chunk = 0, knob_plane_blocks = 0;
     hipGraphExec_t graphs[2][2][4][2] = {};  // [FP64][counting][round order][wake_tol > 0]
     double graph_wake_tol[2][2][4][2] = {};  // the wake_tol each graph was captured with
     bool graph_lean[2][2][4][2] = {};        // and its arithmetic
     float graph_ulps[2][2][4][2] = {};       // and its FP32 settle threshold
+    float graph_wulps[2][2][4][2] = {};      // and its FP32 wake threshold
     // chunks of groups sweeping side by side, each on its own stream with its own graph of plane launches
     static constexpr int kMaxStreams = 8;
     hipStream_t gstream[kMaxStreams] = {};
@@ -838,6 +851,7 @@ struct ts_tableset {
     double cgraph_wake_tol[2][2][4][2][kMaxStreams] = {};
     bool cgraph_lean[2][2][4][2][kMaxStreams] = {};
     float cgraph_ulps[2][2][4][2][kMaxStreams] = {};
+    float cgraph_wulps[2][2][4][2][kMaxStreams] = {};
     int cgraph_groups[2][2][4][2][kMaxStreams][2] = {};  // the groups [first, first + count) each swept
     // streaming rounds, per width (FP32 / FP64 times): tile shape, the task list of every group's tiles
     // in an order where each tile follows its upwind tiles, and the queues and progress of a round
@@ -1028,6 +1042,7 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
     A.planes = t.planes.p;
     A.wake_tol = t.wake_tol[sizeof(R) == 8];
     A.settle_ulps = t.settle_ulps;
+    A.wake_ulps = t.wake_ulps;
     A.nx = t.grid.nx; A.ny = t.grid.ny; A.nz = t.grid.nz; A.N = t.N;
     return A;
 }
@@ -1367,8 +1382,10 @@ hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count, int order)
     double &captured = t.graph_wake_tol[w][count][order][woken];
     bool &captured_lean = t.graph_lean[w][count][order][woken];
     float &captured_ulps = t.graph_ulps[w][count][order][woken];
-    // captured with another tolerance, arithmetic or settle threshold (kernel arguments are captured)
-    if (ge && (captured != t.wake_tol[w] || captured_lean != t.lean[w] || captured_ulps != t.settle_ulps)) {
+    float &captured_wulps = t.graph_wulps[w][count][order][woken];
+    // captured with another tolerance, arithmetic or threshold (kernel arguments are captured)
+    if (ge && (captured != t.wake_tol[w] || captured_lean != t.lean[w] || captured_ulps != t.settle_ulps ||
+               captured_wulps != t.wake_ulps)) {
         TS_HIP(hipGraphExecDestroy(ge));
         ge = nullptr;
     }
@@ -1376,6 +1393,7 @@ hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count, int order)
     captured = t.wake_tol[w];
     captured_lean = t.lean[w];
     captured_ulps = t.settle_ulps;
+    captured_wulps = t.wake_ulps;
     hipStream_t st = t.ctx->stream;
     hipGraph_t gr;
     TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1398,11 +1416,12 @@ hipGraphExec_t chunk_graph(ts_tableset &t, R *T, hipStream_t st, bool count, int
     double &captured = t.cgraph_wake_tol[w][count][order][woken][c];
     bool &captured_lean = t.cgraph_lean[w][count][order][woken][c];
     float &captured_ulps = t.cgraph_ulps[w][count][order][woken][c];
+    float &captured_wulps = t.cgraph_wulps[w][count][order][woken][c];
     int *range = t.cgraph_groups[w][count][order][woken][c];
     // captured with another tolerance, arithmetic or settle threshold, or for other groups (the stream
     // count changed between solves)
     if (ge && (captured != t.wake_tol[w] || captured_lean != t.lean[w] || captured_ulps != t.settle_ulps ||
-               range[0] != g0 || range[1] != gc)) {
+               captured_wulps != t.wake_ulps || range[0] != g0 || range[1] != gc)) {
         TS_HIP(hipGraphExecDestroy(ge));
         ge = nullptr;
     }
@@ -1410,6 +1429,7 @@ hipGraphExec_t chunk_graph(ts_tableset &t, R *T, hipStream_t st, bool count, int
     captured = t.wake_tol[w];
     captured_lean = t.lean[w];
     captured_ulps = t.settle_ulps;
+    captured_wulps = t.wake_ulps;
     range[0] = g0;
     range[1] = gc;
     hipGraph_t gr;
@@ -1599,8 +1619,9 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     t.round_seconds.clear();
     t.sweep_by_width[0] = t.sweep_by_width[1] = 0;
     t.rounds_by_width[0] = t.rounds_by_width[1] = 0;
-    // The FP64 form wakes a node's readers on every drop (Relax's result, bit for bit), and so does the
-    // mixed form's first-order phase; its second-order phase only on drops above mixed_wake_tol().
+    // The FP64 form wakes a node's readers on every drop (Relax's result, bit for bit); the mixed form's
+    // first-order phase on drops above kFp32WakeUlps of the time, its second-order phase on drops above
+    // mixed_wake_tol().
     t.wake_tol[0] = 0.0;
     t.wake_tol[1] = mixed ? mixed_wake_tol() : 0.0;
     // The mixed form's second-order phase uses godunov_lean64 (TOMOSTAR_MIXED_EXACT=1: Relax's arithmetic);
@@ -1611,6 +1632,7 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     t.lean[0] = mixed && lean_first;
     t.lean[1] = mixed && !exact;
     t.settle_ulps = getenv("TOMOSTAR_FP32_SETTLE_ULPS") ? (float)atof(getenv("TOMOSTAR_FP32_SETTLE_ULPS")) : kFp32SettleUlps;
+    t.wake_ulps = !mixed ? 0.0f : getenv("TOMOSTAR_FP32_WAKE_ULPS") ? (float)atof(getenv("TOMOSTAR_FP32_WAKE_ULPS")) : kFp32WakeUlps;
     if (mixed) {
         run_rounds<float>(t, t.T32.p, count, cancel, h_status, false, true);
         eik_mixed_widen<<<blocks_for(nodes_total * kLanes, 256), 256, 0, st>>>(t.T32.p, t.T64.p, nodes_total * kLanes);
