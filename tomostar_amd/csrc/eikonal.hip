@@ -666,7 +666,7 @@ __global__ __launch_bounds__(64 * WPB) void eik_fast_plane_sparse(FastArgs<R> A,
 // in FP64, once every field has settled its first-order phase in FP32).
 __global__ __launch_bounds__(64) void eik_fast_round_end(uint64_t *active, uint64_t *phase1, uint64_t *changed,
                                                          int32_t *state, int32_t *rounds, int cap,
-                                                         int32_t *status, uint8_t *reset, int hold)
+                                                         int32_t *status, uint8_t *reset, int hold, int first_cap)
 {
     const int g = blockIdx.x, lane = threadIdx.x, f = g * 64 + lane;
     int st = state[f];
@@ -674,7 +674,8 @@ __global__ __launch_bounds__(64) void eik_fast_round_end(uint64_t *active, uint6
     bool need_reset = false;
     if (st < 2) {
         int r = ++rounds[2 * f + st];
-        if (!ch) {
+        // first_cap (held first-order phases only): the phase ends after that many rounds, settled or not
+        if (!ch || (hold && st == 0 && first_cap > 0 && r >= first_cap)) {
             if (st == 0) {
                 st = hold ? 5 : 1;
                 need_reset = !hold;
@@ -836,22 +837,35 @@ struct ts_tableset {
     bool lean[2] = {false, false};        // the running solve's FP32 / FP64 sweeps use the mixed form's arithmetic
     float settle_ulps = 4096;             // the running solve's FP32 settle threshold (ulps of the time)
     float wake_ulps = 0;                  // and its FP32 wake threshold (ulps of the time)
+    int first_rounds = 0;                 // mixed form: first-order rounds after which a field is held (0: no
+                                          // cap; TOMOSTAR_FP32_MAX_ROUNDS)
     // launch-shape knobs, read once when the table set is created (graphs captured with them stay valid)
     int knob_group_chunk = 0, knob_plane_blocks = 0;
+    // what a captured graph's kernel arguments hold of the running solve: its tolerances, thresholds
+    // and arithmetic
+    struct SolveKey {
+        double wake_tol = 0;
+        float settle_ulps = 0, wake_ulps = 0;
+        int first_rounds = 0;
+        bool lean = false;
+        bool operator==(const SolveKey &o) const
+        {
+            return wake_tol == o.wake_tol && settle_ulps == o.settle_ulps &&
+                   wake_ulps == o.wake_ulps && first_rounds == o.first_rounds && lean == o.lean;
+        }
+    };
+    SolveKey solve_key(int w) const
+    {
+        return {wake_tol[w], settle_ulps, wake_ulps, first_rounds, lean[w]};
+    }
     hipGraphExec_t graphs[2][2][4][2] = {};  // [FP64][counting][round order][wake_tol > 0]
-    double graph_wake_tol[2][2][4][2] = {};  // the wake_tol each graph was captured with
-    bool graph_lean[2][2][4][2] = {};        // and its arithmetic
-    float graph_ulps[2][2][4][2] = {};       // and its FP32 settle threshold
-    float graph_wulps[2][2][4][2] = {};      // and its FP32 wake threshold
+    SolveKey graph_key[2][2][4][2] = {};     // what each graph was captured with
     // chunks of groups sweeping side by side, each on its own stream with its own graph of plane launches
     static constexpr int kMaxStreams = 8;
     hipStream_t gstream[kMaxStreams] = {};
     hipEvent_t gfork = nullptr, gjoin[kMaxStreams] = {};
     hipGraphExec_t cgraphs[2][2][4][2][kMaxStreams] = {};
-    double cgraph_wake_tol[2][2][4][2][kMaxStreams] = {};
-    bool cgraph_lean[2][2][4][2][kMaxStreams] = {};
-    float cgraph_ulps[2][2][4][2][kMaxStreams] = {};
-    float cgraph_wulps[2][2][4][2][kMaxStreams] = {};
+    SolveKey cgraph_key[2][2][4][2][kMaxStreams] = {};
     int cgraph_groups[2][2][4][2][kMaxStreams][2] = {};  // the groups [first, first + count) each swept
     // streaming rounds, per width (FP32 / FP64 times): tile shape, the task list of every group's tiles
     // in an order where each tile follows its upwind tiles, and the queues and progress of a round
@@ -1125,7 +1139,8 @@ void round_end(ts_tableset &t, hipStream_t st, int hold)
 {
     const int groups = (t.nfields + kLanes - 1) / kLanes;
     eik_fast_round_end<<<groups, 64, 0, st>>>(t.active.p, t.phase1.p, t.changed.p, t.state.p, t.rounds_dev.p,
-                                              sweep_round_cap(t.grid), t.status.p, t.reset.p, hold);
+                                              sweep_round_cap(t.grid), t.status.p, t.reset.p, hold,
+                                              hold ? t.first_rounds : 0);
     dim3 rg(blocks_for(t.N, 256), groups);
     eik_fast_reset_dirty<<<rg, 256, 0, st>>>(t.dirty.p, t.reset.p, t.N);
 }
@@ -1379,21 +1394,14 @@ hipGraphExec_t round_graph(ts_tableset &t, R *T, bool count, int order)
     const int w = sizeof(R) == 8;
     const int woken = t.wake_tol[w] > 0;
     hipGraphExec_t &ge = t.graphs[w][count][order][woken];
-    double &captured = t.graph_wake_tol[w][count][order][woken];
-    bool &captured_lean = t.graph_lean[w][count][order][woken];
-    float &captured_ulps = t.graph_ulps[w][count][order][woken];
-    float &captured_wulps = t.graph_wulps[w][count][order][woken];
-    // captured with another tolerance, arithmetic or threshold (kernel arguments are captured)
-    if (ge && (captured != t.wake_tol[w] || captured_lean != t.lean[w] || captured_ulps != t.settle_ulps ||
-               captured_wulps != t.wake_ulps)) {
+    ts_tableset::SolveKey &captured = t.graph_key[w][count][order][woken];
+    // captured with another tolerance, threshold or arithmetic (kernel arguments are captured)
+    if (ge && !(captured == t.solve_key(w))) {
         TS_HIP(hipGraphExecDestroy(ge));
         ge = nullptr;
     }
     if (ge) return ge;
-    captured = t.wake_tol[w];
-    captured_lean = t.lean[w];
-    captured_ulps = t.settle_ulps;
-    captured_wulps = t.wake_ulps;
+    captured = t.solve_key(w);
     hipStream_t st = t.ctx->stream;
     hipGraph_t gr;
     TS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1413,23 +1421,16 @@ hipGraphExec_t chunk_graph(ts_tableset &t, R *T, hipStream_t st, bool count, int
     const int w = sizeof(R) == 8;
     const int woken = t.wake_tol[w] > 0;
     hipGraphExec_t &ge = t.cgraphs[w][count][order][woken][c];
-    double &captured = t.cgraph_wake_tol[w][count][order][woken][c];
-    bool &captured_lean = t.cgraph_lean[w][count][order][woken][c];
-    float &captured_ulps = t.cgraph_ulps[w][count][order][woken][c];
-    float &captured_wulps = t.cgraph_wulps[w][count][order][woken][c];
+    ts_tableset::SolveKey &captured = t.cgraph_key[w][count][order][woken][c];
     int *range = t.cgraph_groups[w][count][order][woken][c];
-    // captured with another tolerance, arithmetic or settle threshold, or for other groups (the stream
-    // count changed between solves)
-    if (ge && (captured != t.wake_tol[w] || captured_lean != t.lean[w] || captured_ulps != t.settle_ulps ||
-               captured_wulps != t.wake_ulps || range[0] != g0 || range[1] != gc)) {
+    // captured with another tolerance, threshold or arithmetic, or for other groups (the stream count
+    // changed between solves)
+    if (ge && (!(captured == t.solve_key(w)) || range[0] != g0 || range[1] != gc)) {
         TS_HIP(hipGraphExecDestroy(ge));
         ge = nullptr;
     }
     if (ge) return ge;
-    captured = t.wake_tol[w];
-    captured_lean = t.lean[w];
-    captured_ulps = t.settle_ulps;
-    captured_wulps = t.wake_ulps;
+    captured = t.solve_key(w);
     range[0] = g0;
     range[1] = gc;
     hipGraph_t gr;
@@ -1633,6 +1634,7 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     t.lean[1] = mixed && !exact;
     t.settle_ulps = getenv("TOMOSTAR_FP32_SETTLE_ULPS") ? (float)atof(getenv("TOMOSTAR_FP32_SETTLE_ULPS")) : kFp32SettleUlps;
     t.wake_ulps = !mixed ? 0.0f : getenv("TOMOSTAR_FP32_WAKE_ULPS") ? (float)atof(getenv("TOMOSTAR_FP32_WAKE_ULPS")) : kFp32WakeUlps;
+    t.first_rounds = mixed && getenv("TOMOSTAR_FP32_MAX_ROUNDS") ? std::max(0, atoi(getenv("TOMOSTAR_FP32_MAX_ROUNDS"))) : 0;
     if (mixed) {
         run_rounds<float>(t, t.T32.p, count, cancel, h_status, false, true);
         eik_mixed_widen<<<blocks_for(nodes_total * kLanes, 256), 256, 0, st>>>(t.T32.p, t.T64.p, nodes_total * kLanes);
