@@ -238,7 +238,6 @@ struct FastArgs {
     double wake_tol;              // a drop wakes the node's readers when larger than this (0: any drop)
     float settle_ulps;            // FP32 times: a node moved when it dropped by more than this many ulps
     float wake_ulps;              // FP32 times: a drop wakes the node's readers when larger than this many ulps
-    int dir_wake;                 // a drop wakes only the readers it can lower (reader_mask)
     int nx, ny, nz, N;
 };
 
@@ -440,7 +439,7 @@ __device__ __forceinline__ void drop_tests(const FastArgs<R> &A, R cur, R next, 
 // node lies in the grid.
 template <typename R, bool kSecond>
 __device__ __forceinline__ void wake_readers(const FastArgs<R> &A, size_t gn, int node, int i, int j, int k,
-                                             int lane, int wake_off, uint32_t which = 0xfffu)
+                                             int lane, int wake_off)
 {
     const int pos[3] = {i, j, k}, n[3] = {A.nx, A.ny, A.nz};
     uint32_t in = 0;
@@ -449,31 +448,7 @@ __device__ __forceinline__ void wake_readers(const FastArgs<R> &A, size_t gn, in
         in |= (uint32_t)((pos[ax] >= 1) | (pos[ax] >= 2) << 1 | (pos[ax] + 1 < n[ax]) << 2 |
                          (pos[ax] + 2 < n[ax]) << 3) << (4 * ax);
     if (!kSecond) in &= 0x555u;
-    in &= which;
     if (((uint64_t)in >> lane) & 1) A.dirty[gn + node + wake_off] = 1;
-}
-
-// The readers (bit 4a + w, as wake_readers) a drop of the node to `next` can lower, for the lanes that
-// dropped it: those whose time is not below next. An update's result is at least every upwind time it
-// uses (the causality checks of the Godunov update), so a reader already below the node's new time
-// cannot take a lower time through it, as its first neighbour on the axis or as the one beyond; the
-// margin covers the rounding of the result (a few ulps). The reader times are the neighbour times the
-// update loaded: the nodes a node reads are the nodes that read it.
-template <typename R, bool kSecond>
-__device__ __forceinline__ uint32_t reader_mask(const NodeLoad<R> &L, R next, bool dropped)
-{
-    const R lim = next - next * (sizeof(R) == 4 ? (R)1e-6 : (R)1e-12);
-    uint32_t m = 0;
-#pragma unroll
-    for (int ax = 0; ax < 3; ax++) {
-        m |= (__ballot(dropped && !(L.mn[ax] < lim)) != 0 ? 1u : 0u) << (4 * ax);
-        m |= (__ballot(dropped && !(L.pl[ax] < lim)) != 0 ? 4u : 0u) << (4 * ax);
-        if constexpr (kSecond) {
-            m |= (__ballot(dropped && !(L.m2[ax] < lim)) != 0 ? 2u : 0u) << (4 * ax);
-            m |= (__ballot(dropped && !(L.p2[ax] < lim)) != 0 ? 8u : 0u) << (4 * ax);
-        }
-    }
-    return m;
 }
 
 // The update itself, from what load_node read: the new time, and the node's readers woken when it
@@ -496,10 +471,7 @@ __device__ __forceinline__ uint64_t finish_node(const FastArgs<R> &A, const Node
         A.T[(gn + node) * kLanes + lane] = next;
         drop_tests(A, L.cur, next, dropped, moved);
     }
-    if (__ballot(dropped)) {
-        const uint32_t which = A.dir_wake ? reader_mask<R, kSecond>(L, next, dropped) : 0xfffu;
-        wake_readers<R, kSecond>(A, gn, node, L.i, L.j, L.k, lane, wake_off, which);
-    }
+    if (__ballot(dropped)) wake_readers<R, kSecond>(A, gn, node, L.i, L.j, L.k, lane, wake_off);
     const uint64_t mv = __ballot(moved);
     if constexpr (sizeof(R) == 8) {
         // FP64 times: into the group's changed mask at once, when the mask (read at the device's scope)
@@ -538,7 +510,7 @@ __device__ __forceinline__ uint64_t stream_update(const FastArgs<R> &A, const No
     } else {
         next = L.cur;
     }
-    uint64_t woke = __ballot(dropped);
+    const uint64_t woke = __ballot(dropped);
     if (woke) {
         const int pos[3] = {L.i, L.j, L.k}, n[3] = {A.nx, A.ny, A.nz};
         uint32_t in = 0;
@@ -547,7 +519,6 @@ __device__ __forceinline__ uint64_t stream_update(const FastArgs<R> &A, const No
             in |= (uint32_t)((pos[ax] >= 1) | (pos[ax] >= 2) << 1 | (pos[ax] + 1 < n[ax]) << 2 |
                              (pos[ax] + 2 < n[ax]) << 3) << (4 * ax);
         if (!kSecond) in &= 0x555u;
-        if (A.dir_wake) in &= reader_mask<R, kSecond>(L, next, dropped);
         if (((uint64_t)in >> lane) & 1) {
             uint8_t *f = &A.dirty[gn + L.node + wake_off];
             if (dev) st_dev(f, (uint8_t)1);
@@ -866,7 +837,6 @@ struct ts_tableset {
     bool lean[2] = {false, false};        // the running solve's FP32 / FP64 sweeps use the mixed form's arithmetic
     float settle_ulps = 4096;             // the running solve's FP32 settle threshold (ulps of the time)
     float wake_ulps = 0;                  // and its FP32 wake threshold (ulps of the time)
-    int dir_wake = 0;                     // the running solve wakes only the readers a drop can lower
     int first_rounds = 0;                 // mixed form: first-order rounds after which a field is held (0: no
                                           // cap; TOMOSTAR_FP32_MAX_ROUNDS)
     // launch-shape knobs, read once when the table set is created (graphs captured with them stay valid)
@@ -876,18 +846,17 @@ struct ts_tableset {
     struct SolveKey {
         double wake_tol = 0;
         float settle_ulps = 0, wake_ulps = 0;
-        int first_rounds = 0, dir_wake = 0;
+        int first_rounds = 0;
         bool lean = false;
         bool operator==(const SolveKey &o) const
         {
             return wake_tol == o.wake_tol && settle_ulps == o.settle_ulps &&
-                   wake_ulps == o.wake_ulps && first_rounds == o.first_rounds && dir_wake == o.dir_wake &&
-                   lean == o.lean;
+                   wake_ulps == o.wake_ulps && first_rounds == o.first_rounds && lean == o.lean;
         }
     };
     SolveKey solve_key(int w) const
     {
-        return {wake_tol[w], settle_ulps, wake_ulps, first_rounds, dir_wake, lean[w]};
+        return {wake_tol[w], settle_ulps, wake_ulps, first_rounds, lean[w]};
     }
     hipGraphExec_t graphs[2][2][4][2] = {};  // [FP64][counting][round order][wake_tol > 0]
     SolveKey graph_key[2][2][4][2] = {};     // what each graph was captured with
@@ -1088,7 +1057,6 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
     A.wake_tol = t.wake_tol[sizeof(R) == 8];
     A.settle_ulps = t.settle_ulps;
     A.wake_ulps = t.wake_ulps;
-    A.dir_wake = t.dir_wake;
     A.nx = t.grid.nx; A.ny = t.grid.ny; A.nz = t.grid.nz; A.N = t.N;
     return A;
 }
@@ -1666,7 +1634,6 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     t.lean[1] = mixed && !exact;
     t.settle_ulps = getenv("TOMOSTAR_FP32_SETTLE_ULPS") ? (float)atof(getenv("TOMOSTAR_FP32_SETTLE_ULPS")) : kFp32SettleUlps;
     t.wake_ulps = !mixed ? 0.0f : getenv("TOMOSTAR_FP32_WAKE_ULPS") ? (float)atof(getenv("TOMOSTAR_FP32_WAKE_ULPS")) : kFp32WakeUlps;
-    t.dir_wake = mixed && getenv("TOMOSTAR_DIR_WAKE") ? atoi(getenv("TOMOSTAR_DIR_WAKE")) : 0;
     t.first_rounds = mixed && getenv("TOMOSTAR_FP32_MAX_ROUNDS") ? std::max(0, atoi(getenv("TOMOSTAR_FP32_MAX_ROUNDS"))) : 0;
     if (mixed) {
         run_rounds<float>(t, t.T32.p, count, cancel, h_status, false, true);
