@@ -576,8 +576,10 @@ __device__ __forceinline__ void plane_nodes(const FastArgs<R> &A, int dir, int o
 // updates plane nodes a, a + W, a + 2W, ... (W the waves of a group's row of blocks), so a launch
 // is a few resident waves per SIMD rather than one short-lived workgroup per node. kFlag: the dirty
 // flag read first (rounds where most nodes are clean) or with every other load (where most are not).
+// The all-loads variant is held to 8 waves per SIMD (its FP64 forms otherwise take 101 scalar registers
+// and 7 waves; held, 78 and no spills): the round it runs, where every node is dirty, is latency-bound.
 template <typename R, int WPB, bool kCount, bool kOff32, int kFlag, bool kLean>
-__global__ __launch_bounds__(64 * WPB) void eik_fast_plane(FastArgs<R> A, int dir, int off, int cnt, int g0)
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(kFlag == kFlagWithLoads ? 8 : 1))) void eik_fast_plane(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
     const int g = g0 + blockIdx.y;
@@ -633,9 +635,10 @@ __device__ __forceinline__ void sparse_nodes(const FastArgs<R> &A, int dir, int 
 
 // Rounds in which most nodes are clean: each wavefront reads the dirty flags of M consecutive plane
 // nodes at once (lanes 0 .. M-1) and then updates its dirty ones in plane order, so a clean stretch of
-// the plane costs one load per M nodes and M times fewer waves are launched.
+// the plane costs one load per M nodes and M times fewer waves are launched. FP32: held to 7 waves per
+// SIMD (72 vector registers instead of 76 and 6 waves, no spills; 8 waves would spill).
 template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean, int K>
-__global__ __launch_bounds__(64 * WPB) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 7 : 1))) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
     const int g = g0 + blockIdx.y;
