@@ -253,6 +253,14 @@ __device__ inline T at_off(const T *base, uint32_t byte_off)
 // memory once; read it first, so a clean node costs one load; or know it is set (checked beforehand).
 enum FlagRead { kFlagWithLoads = 0, kFlagFirst = 1, kFlagKnown = 2 };
 
+// Waves per SIMD the sparse sweep kernel is held to (1: no hold).
+#ifndef TS_SPARSE_WAVES32
+#define TS_SPARSE_WAVES32 7
+#endif
+#ifndef TS_SPARSE_WAVES64
+#define TS_SPARSE_WAVES64 1
+#endif
+
 // Grid node of packed plane entry pk (i' | j' << 11 | k' << 22 in the ordering's mirrored coordinates),
 // and its unmirrored coordinates.
 template <typename R>
@@ -638,7 +646,7 @@ __device__ __forceinline__ void sparse_nodes(const FastArgs<R> &A, int dir, int 
 // the plane costs one load per M nodes and M times fewer waves are launched. FP32: held to 7 waves per
 // SIMD (72 vector registers instead of 76 and 6 waves, no spills; 8 waves would spill).
 template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean, int K>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 7 : 1))) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? TS_SPARSE_WAVES32 : TS_SPARSE_WAVES64))) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
     const int g = g0 + blockIdx.y;
@@ -1074,13 +1082,19 @@ constexpr int kStreamShapes = 2;  // tile shapes per width (with_shape)
 #ifndef TS_INFLIGHT32
 #define TS_INFLIGHT32 4
 #endif
+#ifndef TS_SPARSE_NODES64
+#define TS_SPARSE_NODES64 2
+#endif
+#ifndef TS_INFLIGHT64
+#define TS_INFLIGHT64 1
+#endif
 // Flags read per wave of a sparse launch (M) and dirty nodes whose loads are in flight together (K): FP32
 // waves carry the loads of K first-order nodes within the registers of full occupancy, FP64 waves (twelve
 // FP64 neighbours) one.
 template <typename R>
-constexpr int kSparseNodes = sizeof(R) == 4 ? TS_SPARSE_NODES32 : 2;
+constexpr int kSparseNodes = sizeof(R) == 4 ? TS_SPARSE_NODES32 : TS_SPARSE_NODES64;
 template <typename R>
-constexpr int kInFlight = sizeof(R) == 4 ? TS_INFLIGHT32 : 1;
+constexpr int kInFlight = sizeof(R) == 4 ? TS_INFLIGHT32 : TS_INFLIGHT64;
 
 // Cap on the blocks of a plane launch over all its groups (TOMOSTAR_PLANE_BLOCKS): waves stride over
 // the plane beyond it. Measured on the 400-field case: fewer, longer-lived waves are slower (the
