@@ -1072,7 +1072,10 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
     return A;
 }
 
-constexpr int kWPB = 4;
+#ifndef TS_WPB
+#define TS_WPB 2
+#endif
+constexpr int kWPB = TS_WPB;  // waves per block of the plane kernels
 constexpr int kSparse = 2;  // round order after kFlagWithLoads, kFlagFirst: kSparseNodes flags per wave, then the dirty nodes
 constexpr int kStream = 3;  // the streaming sweep: one persistent launch per ordering over column tiles
 constexpr int kStreamShapes = 2;  // tile shapes per width (with_shape)
