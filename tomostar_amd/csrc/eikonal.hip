@@ -300,6 +300,14 @@ struct NodeLoad {
     R wlon0, wlon1, wlat0, wlat1;  // spacing weights of the x and y axes (first, second order)
 };
 
+// Plane entry q, for a wave-uniform q: a scalar load (the plane lists are never written by a sweep, so
+// the constant address space, read through the scalar cache, holds them).
+template <typename R>
+__device__ __forceinline__ uint32_t plane_entry(const FastArgs<R> &A, int q)
+{
+    return ((const __attribute__((address_space(4))) uint32_t *)A.planes)[q];
+}
+
 // The lane whose field's times a lane loads: its own while its field relaxes this round, else the
 // first such lane of the wave. A lane whose field is settled, done or unused never stores what it
 // loads (node_update returns its current time with on false), so its loads only cost HBM rows: sent
@@ -574,7 +582,7 @@ __device__ __forceinline__ void plane_nodes(const FastArgs<R> &A, int dir, int o
     uint64_t mv = 0;
     for (; a < cnt; a += stride) {
         NodeLoad<R> L;
-        load_node<R, kOff32, kFlag, kSecond, false>(A, dir, A.planes[off + a], g, ll, mdl, L);
+        load_node<R, kOff32, kFlag, kSecond, false>(A, dir, plane_entry(A, off + a), g, ll, mdl, L);
         mv |= finish_node<R, kCount, kLean, kSecond>(A, L, g, act, ph, lane, blockIdx.x, wake_off);
     }
     mark_changed(&A.changed[g], mv, lane);
@@ -599,7 +607,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(kFlag 
     const int per_xcd = (gridDim.x + 7) / 8;
     const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     const int stride = 8 * per_xcd * WPB;
-    const int a = lb * WPB + (threadIdx.x >> 6);
+    const int a = lb * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform: scalar loads of the plane entry
     const uint64_t ph = A.phase1[g];
     // The mixed form's FP64 kernels (kLean) run its second-order phase only: one body, the kernel's code
     // half the size (the second-order body is right for any wave, only slower for a first-order one).
@@ -656,6 +664,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     const int a0 = (lb * WPB + (threadIdx.x >> 6)) * M;
     if (a0 >= cnt) return;
+    // lanes 0 .. M-1 read the plane entries and their flags side by side (vector loads: as scalar loads
+    // the M nodes' coordinates are worked out one after another, and the sparse rounds took longer)
     uint32_t pk = 0;
     bool dirty = false;
     if (lane < M && a0 + lane < cnt) {
