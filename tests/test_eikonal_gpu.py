@@ -258,6 +258,36 @@ def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
     ts.release()
 
 
+def test_first_order_wake_threshold(device, oracle, monkeypatch):
+    """The mixed form's first-order phase wakes a node's readers only on drops above
+    TOMOSTAR_FP32_WAKE_ULPS ulps of its time (16 by default): the times stay within the bar of Relax
+    whatever the threshold, a threshold of 0 (every drop) gives the same times as one of a quarter ulp
+    (no drop of an FP32 time is smaller than half an ulp of it), and the FP64 form ignores the setting (Relax to the bit under every threshold)."""
+    from tomostar_amd import eikonal as E
+    from tomostar_amd import workloads as W
+
+    wl = _checker_case(11, nx=37, ny=33, nz=19, nst=36)  # 72 fields: two groups, the second of 8 fields
+    seeds, mof = W.station_seed_sets(wl, E.point_source)
+    models = [wl.slowness_p, wl.slowness_s]
+    ref = _relax_fields(oracle, wl.grid, models, seeds, mof)
+    ts = E.TableSet(device, wl.grid, models, seeds, mof)
+    out = {}
+    for ulps in ("0", "0.25", "16", "256"):
+        monkeypatch.setenv("TOMOSTAR_FP32_WAKE_ULPS", ulps)
+        print(f"wake threshold {ulps} ulps", flush=True)
+        rc, why = ts.solve("fast_fp32")
+        assert rc == 0, why
+        out[ulps] = ts.read()
+        err = _rel_err(out[ulps], ref, seeds)
+        print(f"  max relative error against Relax {err:.3e}")
+        assert err <= FP32_TOL
+        rc, why = ts.solve("fast_fp64")
+        assert rc == 0, why
+        assert np.array_equal(ts.read(), ref), ulps
+    assert np.array_equal(out["0"], out["0.25"])
+    ts.release()
+
+
 def _suggest_batch_rule(nodes, global_mem):
     """EikonalOpenCl.SuggestBatch (EikonalOpenCl.cs:121-132) with Real = 8 (an FP64 device) and MaxAlloc =
     the device's memory (HIP sets no separate allocation limit); C#'s long division truncates."""
