@@ -840,6 +840,7 @@ struct ts_tableset {
     dbuf<float> tab32;                    // [fields][N] stored tables
     dbuf<double> tab64;                   // [fields][N] when the last solve ran in FP64
     bool have64 = false, solved = false;
+    bool pending64 = false;               // the fast form left its FP64 times in T64: tab64 on the first FP64 read
     std::vector<int32_t> rounds;          // 2 per field
     double evaluations = 0, evaluations32 = 0, evaluations64 = 0, seconds = 0, sweep_seconds = 0;
     double sweep_by_width[2] = {0, 0};  // sweep-graph device seconds on FP32 / FP64 times
@@ -1054,6 +1055,7 @@ void solve_plain(ts_tableset &t, const volatile int32_t *cancel)
         }
     }
     t.have64 = true;
+    t.pending64 = false;
 }
 
 // ---- fast form
@@ -1677,7 +1679,9 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
     }
     run_rounds<double>(t, t.T64.p, count, cancel, h_status, mixed, mixed);
     dim3 ug(blocks_for(N, 64), groups);
-    eik_fast_unpack<double><<<ug, 256, 0, st>>>(t.T64.p, N, t.field_of_slot_dev.p, t.tab32.p, t.tab64.p);
+    // the stored tables are FP32 (TravelTimeTable's float32 tables); the FP64 times stay in T64 until a
+    // caller reads them in FP64 (ts_tables_read), which unpacks them then
+    eik_fast_unpack<double><<<ug, 256, 0, st>>>(t.T64.p, N, t.field_of_slot_dev.p, t.tab32.p, nullptr);
     TS_HIP(hipGetLastError());
     std::vector<int32_t> rounds(2 * slots);
     TS_HIP(hipMemcpyAsync(rounds.data(), t.rounds_dev.p, rounds.size() * 4, hipMemcpyDeviceToHost, st));
@@ -1696,7 +1700,8 @@ void solve_fast(ts_tableset &t, int mode, const volatile int32_t *cancel, bool c
         t.evaluations32 = e32;
         t.evaluations64 = e64;
     }
-    t.have64 = true;
+    t.have64 = false;
+    t.pending64 = true;
 }
 
 }  // namespace
@@ -1933,6 +1938,17 @@ int ts_tables_read(ts_tableset *t, int first, int count, double *out)
     return guarded(nullptr, 0, [&]() -> int {
         TS_HIP(hipSetDevice(t->ctx->device));
         const size_t n = (size_t)count * t->N, at = (size_t)first * t->N;
+        if (t->pending64) {
+            // the last fast solve's FP64 times, field-major, once
+            const int groups = (t->nfields + kLanes - 1) / kLanes;
+            dim3 ug(blocks_for(t->N, 64), groups);
+            eik_fast_unpack<double><<<ug, 256, 0, t->ctx->stream>>>(t->T64.p, t->N, t->field_of_slot_dev.p, nullptr,
+                                                                    t->tab64.p);
+            TS_HIP(hipGetLastError());
+            TS_HIP(hipStreamSynchronize(t->ctx->stream));
+            t->pending64 = false;
+            t->have64 = true;
+        }
         if (t->have64) {
             TS_HIP(hipMemcpy(out, t->tab64.p + at, n * 8, hipMemcpyDeviceToHost));
         } else {
