@@ -1344,8 +1344,8 @@ void launch_planes_as(ts_tableset &t, R *T, hipStream_t st, bool count, int orde
 // Streams the groups sweep on side by side when rounds run as graphs (TOMOSTAR_GROUP_STREAMS overrides;
 // 0 or 1: one stream, every group in every launch). The groups are independent fields, so each chunk of
 // them runs its own graph of plane launches on its own stream between a fork and a join, the results
-// unchanged to the bit. Measured on 400 fields (7 groups) of 201 x 201 x 101: one stream 1.179 s per
-// mixed solve, 2 streams 1.090 s, 3 streams 1.086 s, 4 streams 1.32 s (with the solve's stream, more
+// unchanged to the bit. Measured on 400 fields (7 groups) of 201 x 201 x 101: one stream 0.802 s per
+// mixed solve, 2 streams 0.723 s, 3 streams 0.716 s, 4 streams 0.928 s (with the solve's stream, more
 // streams than the 4 hardware queues hold apart). The plane launches of two chunks overlap: one
 // chunk's launch gaps are filled by the other's waves.
 int group_streams(int groups, bool graphs)
