@@ -226,15 +226,22 @@ __global__ __launch_bounds__(kMgBlock) void mg_plane(MgArgs<R> A, const MgSeg *_
 {
     __shared__ MgSeg sg;
     __shared__ unsigned block_evals;
-    if (threadIdx.x == 0) {
-        block_evals = 0;
-        int lo = 0, hi = nsegs - 1;
-        while (lo < hi) {  // the last segment whose first block is <= blockIdx.x
-            const int mid = (lo + hi + 1) >> 1;
-            if (segs[mid].block_start <= (int)blockIdx.x) lo = mid;
-            else hi = mid - 1;
+    if (threadIdx.x < kWave) {
+        // the last segment whose first block is <= blockIdx.x: the first wave reads 64 segment starts at
+        // a time and ballots (one wait for memory per 64 segments, where a binary search by one thread
+        // waits once per halving)
+        const int lane = threadIdx.x;
+        int found = 0;
+        for (int c0 = 0; c0 < nsegs; c0 += kWave) {
+            const int q = c0 + lane;
+            const uint64_t le = __ballot(q < nsegs && segs[q].block_start <= (int)blockIdx.x);
+            if (le) found = c0 + 63 - __builtin_clzll(le);
+            if (le != ~0ull) break;  // starts ascend: a segment beyond this block ends the search
         }
-        sg = segs[lo];
+        if (lane == 0) {
+            block_evals = 0;
+            sg = segs[found];
+        }
     }
     __syncthreads();
     const int q = sg.problem;
