@@ -218,6 +218,15 @@ def test_full_size_batched_mixed_against_fp64(device):
     err = _rel_err(dev, ref, [seeds[f] for f in picks])
     print(f"max relative error of the mixed form over {len(picks)} fields: {err:.3e}")
     assert err < FP32_TOL
+    # reads large enough to go through the pinned staging buffers (whole chunks and a partial one)
+    # return what field-by-field reads return
+    all32 = ts.read_f32(0, len(seeds))
+    for f in (0, 63, 64, 200, 399):
+        assert np.array_equal(all32[f], ts.read_f32(f, 1)[0], equal_nan=True), f
+    del all32
+    some64 = ts.read(0, 40)
+    for f in (0, 17, 39):
+        assert np.array_equal(some64[f], ts.read(f, 1)[0], equal_nan=True), f
     ts.release()
 
 

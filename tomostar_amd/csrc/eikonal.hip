@@ -24,7 +24,9 @@
 //     to the last bit.
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <memory>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -815,6 +817,61 @@ __global__ void eik_f32_to_f64(const float *in, double *out, size_t n)
 
 using namespace ts;
 
+// Device-to-host copies of tables into a caller's (pageable) array: through two pinned staging buffers,
+// the device's copy of one chunk overlapping the host threads' copy of the one before. Into pageable
+// memory the runtime stages on its own at ~10 GB/s (400 float32 tables of 201 x 201 x 101: 0.63 s).
+struct HostStage {
+    static constexpr size_t kChunk = 128u << 20;
+    char *pin[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    HostStage() = default;
+    HostStage(const HostStage &) = delete;
+    HostStage &operator=(const HostStage &) = delete;
+    ~HostStage()
+    {
+        for (int b = 0; b < 2; b++) {
+            if (pin[b]) (void)hipHostFree(pin[b]);
+            if (done[b]) (void)hipEventDestroy(done[b]);
+        }
+    }
+};
+
+void copy_to_host(std::unique_ptr<HostStage> &hs, hipStream_t st, void *dst, const void *src, size_t bytes)
+{
+    if (bytes < 2 * HostStage::kChunk) {  // small: the runtime's own copy
+        TS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return;
+    }
+    if (!hs) {
+        hs = std::make_unique<HostStage>();
+        for (int b = 0; b < 2; b++) {
+            TS_HIP(hipHostMalloc((void **)&hs->pin[b], HostStage::kChunk, hipHostMallocDefault));
+            TS_HIP(hipEventCreateWithFlags(&hs->done[b], hipEventDisableTiming));
+        }
+    }
+    const size_t chunk = HostStage::kChunk, nchunks = (bytes + chunk - 1) / chunk;
+    const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    auto host_copy = [&](size_t c) {  // chunk c, once the device has written it
+        const size_t off = c * chunk, len = std::min(chunk, bytes - off), part = (len + nthreads - 1) / nthreads;
+        TS_HIP(hipEventSynchronize(hs->done[c % 2]));
+        const char *from = hs->pin[c % 2];
+        char *to = (char *)dst + off;
+        std::vector<std::thread> th;
+        for (size_t lo = part; lo < len; lo += part)
+            th.emplace_back([=] { std::memcpy(to + lo, from + lo, std::min(part, len - lo)); });
+        std::memcpy(to, from, std::min(part, len));
+        for (auto &x : th) x.join();
+    };
+    for (size_t c = 0; c < nchunks; c++) {
+        // buffer c % 2 last held chunk c - 2, whose host copy has finished
+        const size_t off = c * chunk, len = std::min(chunk, bytes - off);
+        TS_HIP(hipMemcpyAsync(hs->pin[c % 2], (const char *)src + off, len, hipMemcpyDeviceToHost, st));
+        TS_HIP(hipEventRecord(hs->done[c % 2], st));
+        if (c >= 1) host_copy(c - 1);
+    }
+    host_copy(nchunks - 1);
+}
+
 struct ts_tableset {
     ts_context *ctx = nullptr;
     Grid grid{};
@@ -841,6 +898,7 @@ struct ts_tableset {
     dbuf<double> tab64;                   // [fields][N] when the last solve ran in FP64
     bool have64 = false, solved = false;
     bool pending64 = false;               // the fast form left its FP64 times in T64: tab64 on the first FP64 read
+    std::unique_ptr<HostStage> stage;     // pinned staging of large reads (copy_to_host)
     std::vector<int32_t> rounds;          // 2 per field
     double evaluations = 0, evaluations32 = 0, evaluations64 = 0, seconds = 0, sweep_seconds = 0;
     double sweep_by_width[2] = {0, 0};  // sweep-graph device seconds on FP32 / FP64 times
@@ -1950,7 +2008,7 @@ int ts_tables_read(ts_tableset *t, int first, int count, double *out)
             t->have64 = true;
         }
         if (t->have64) {
-            TS_HIP(hipMemcpy(out, t->tab64.p + at, n * 8, hipMemcpyDeviceToHost));
+            copy_to_host(t->stage, t->ctx->stream, out, t->tab64.p + at, n * 8);
         } else {
             // widen on the device in slabs, so the host never holds a second copy
             const size_t slab = std::min<size_t>(n, 32u << 20);
@@ -1972,7 +2030,7 @@ int ts_tables_read_f32(ts_tableset *t, int first, int count, float *out)
     std::lock_guard<std::mutex> lk(t->gate);
     return guarded(nullptr, 0, [&]() -> int {
         TS_HIP(hipSetDevice(t->ctx->device));
-        TS_HIP(hipMemcpy(out, t->tab32.p + (size_t)first * t->N, (size_t)count * t->N * 4, hipMemcpyDeviceToHost));
+        copy_to_host(t->stage, t->ctx->stream, out, t->tab32.p + (size_t)first * t->N, (size_t)count * t->N * 4);
         return TS_OK;
     });
 }
