@@ -24,9 +24,7 @@
 //     to the last bit.
 #include <algorithm>
 #include <chrono>
-#include <cstring>
 #include <memory>
-#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -816,61 +814,6 @@ __global__ void eik_f32_to_f64(const float *in, double *out, size_t n)
 // ================================================================================== host side
 
 using namespace ts;
-
-// Device-to-host copies of tables into a caller's (pageable) array: through two pinned staging buffers,
-// the device's copy of one chunk overlapping the host threads' copy of the one before. Into pageable
-// memory the runtime stages on its own at ~10 GB/s (400 float32 tables of 201 x 201 x 101: 0.63 s).
-struct HostStage {
-    static constexpr size_t kChunk = 128u << 20;
-    char *pin[2] = {nullptr, nullptr};
-    hipEvent_t done[2] = {nullptr, nullptr};
-    HostStage() = default;
-    HostStage(const HostStage &) = delete;
-    HostStage &operator=(const HostStage &) = delete;
-    ~HostStage()
-    {
-        for (int b = 0; b < 2; b++) {
-            if (pin[b]) (void)hipHostFree(pin[b]);
-            if (done[b]) (void)hipEventDestroy(done[b]);
-        }
-    }
-};
-
-void copy_to_host(std::unique_ptr<HostStage> &hs, hipStream_t st, void *dst, const void *src, size_t bytes)
-{
-    if (bytes < 2 * HostStage::kChunk) {  // small: the runtime's own copy
-        TS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-        return;
-    }
-    if (!hs) {
-        hs = std::make_unique<HostStage>();
-        for (int b = 0; b < 2; b++) {
-            TS_HIP(hipHostMalloc((void **)&hs->pin[b], HostStage::kChunk, hipHostMallocDefault));
-            TS_HIP(hipEventCreateWithFlags(&hs->done[b], hipEventDisableTiming));
-        }
-    }
-    const size_t chunk = HostStage::kChunk, nchunks = (bytes + chunk - 1) / chunk;
-    const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-    auto host_copy = [&](size_t c) {  // chunk c, once the device has written it
-        const size_t off = c * chunk, len = std::min(chunk, bytes - off), part = (len + nthreads - 1) / nthreads;
-        TS_HIP(hipEventSynchronize(hs->done[c % 2]));
-        const char *from = hs->pin[c % 2];
-        char *to = (char *)dst + off;
-        std::vector<std::thread> th;
-        for (size_t lo = part; lo < len; lo += part)
-            th.emplace_back([=] { std::memcpy(to + lo, from + lo, std::min(part, len - lo)); });
-        std::memcpy(to, from, std::min(part, len));
-        for (auto &x : th) x.join();
-    };
-    for (size_t c = 0; c < nchunks; c++) {
-        // buffer c % 2 last held chunk c - 2, whose host copy has finished
-        const size_t off = c * chunk, len = std::min(chunk, bytes - off);
-        TS_HIP(hipMemcpyAsync(hs->pin[c % 2], (const char *)src + off, len, hipMemcpyDeviceToHost, st));
-        TS_HIP(hipEventRecord(hs->done[c % 2], st));
-        if (c >= 1) host_copy(c - 1);
-    }
-    host_copy(nchunks - 1);
-}
 
 struct ts_tableset {
     ts_context *ctx = nullptr;

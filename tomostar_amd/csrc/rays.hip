@@ -683,8 +683,9 @@ int ts_rays_rows(ts_rays *r, int32_t *counts, int64_t *row_start, int32_t *nodes
         if (counts) std::copy(r->counts.begin(), r->counts.end(), counts);
         if (row_start) std::copy(r->row_start.begin(), r->row_start.end(), row_start);
         const int64_t nnz = r->row_start.empty() ? 0 : r->row_start.back();
-        if (nnz > 0 && nodes) TS_HIP(hipMemcpy(nodes, r->nodes.p, nnz * 4, hipMemcpyDeviceToHost));
-        if (nnz > 0 && lengths) TS_HIP(hipMemcpy(lengths, r->lengths.p, nnz * 4, hipMemcpyDeviceToHost));
+        if (nnz > 0 && (nodes || lengths)) TS_HIP(hipSetDevice(r->ctx->device));
+        if (nnz > 0 && nodes) copy_to_host(r->stage, exec_stream(r->ctx), nodes, r->nodes.p, nnz * 4);
+        if (nnz > 0 && lengths) copy_to_host(r->stage, exec_stream(r->ctx), lengths, r->lengths.p, nnz * 4);
         if (tcalc_grad) std::copy(r->tcalc_grad.begin(), r->tcalc_grad.end(), tcalc_grad);
         if (path_points) std::copy(r->path_points.begin(), r->path_points.end(), path_points);
         return TS_OK;

@@ -20,5 +20,6 @@ struct ts_rays {
     std::vector<double> outside;      // per arrival: time of the pieces outside the inversion grid (0: none)
     bool extended = false;            // traced with a background (pieces outside the grid timed, not attributed)
     double seconds = 0;
+    std::unique_ptr<ts::HostStage> stage;  // pinned staging of large row reads (copy_to_host)
 };
 
