@@ -143,7 +143,7 @@ int ts_tables_stats(ts_tableset *t, double *evaluations_fp32, double *evaluation
 int ts_tables_sweep_stats(ts_tableset *t, int32_t rounds[2], int32_t *plane_launches_per_round, double seconds[2]);
 /* The rounds of the last fast solve, in order: kind[q] = 16 for a round on FP64 times (0: FP32) | how
  * the round swept (0 dirty flag read with every load, 1 flag first, 2 a few flags per wave, 3 the
- * streaming sweep over column tiles), seconds[q] its device seconds. Fills min(max, rounds) entries and
+ * streaming sweep over column tiles, 4 more flags per wave), seconds[q] its device seconds. Fills min(max, rounds) entries and
  * returns the number of rounds. No counterpart in the reference (diagnostics of the device schedule). */
 int ts_tables_round_log(ts_tableset *t, int max, int32_t *kind, double *seconds);
 void ts_tables_release(ts_tableset *t);

@@ -231,7 +231,7 @@ def test_full_size_batched_mixed_against_fp64(device):
 
 
 def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
-    """How a round reads the dirty flags (every load at once, the flag first, a few flags per wave) and
+    """How a round reads the dirty flags (every load at once, the flag first, a few or more flags per wave) and
     how many streams the groups sweep on change when loads are issued, not which nodes update nor their
     arithmetic: no node of a plane reads or wakes another node of the same plane, and the groups are
     independent fields. The mixed form's times are bit-identical under every schedule, also when the
@@ -250,13 +250,15 @@ def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
         {"TOMOSTAR_SPARSE_FROM": "99", "TOMOSTAR_SECOND_SPARSE_FROM": "99"},  # flag first, no sparse rounds
         {"TOMOSTAR_FP64_FLAG_FIRST": "0"},  # second-order phase sparse from its round 0 (no all-loads round)
         {"TOMOSTAR_SECOND_SPARSE_FROM": "3"},  # second-order rounds 1-2 flag first
+        {"TOMOSTAR_WIDE_FROM": "1", "TOMOSTAR_SECOND_WIDE_FROM": "1"},  # more flags per wave from round 1
+        {"TOMOSTAR_SECOND_WIDE_FROM": "99"},  # never more flags per wave
         {"TOMOSTAR_GROUP_STREAMS": "1"},
         {"TOMOSTAR_GROUP_STREAMS": "2"},  # chunk 1 now holds groups 1-2 (with 3 streams: group 1)
         {},
     ]
     for env in schedules:
         for k in ("TOMOSTAR_SPARSE_FROM", "TOMOSTAR_SECOND_SPARSE_FROM", "TOMOSTAR_FP64_FLAG_FIRST",
-                  "TOMOSTAR_GROUP_STREAMS"):
+                  "TOMOSTAR_GROUP_STREAMS", "TOMOSTAR_WIDE_FROM", "TOMOSTAR_SECOND_WIDE_FROM"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -264,6 +266,9 @@ def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
         rc, why = ts.solve("fast_fp32")
         assert rc == 0, why
         assert np.array_equal(ts.read(), base, equal_nan=True), env
+        if "TOMOSTAR_WIDE_FROM" in env:  # both phases' rounds from 1 on read more flags per wave
+            orders = [(w, o) for w, o, _ in ts.round_log()]
+            assert (32, 4) in orders and (64, 4) in orders, orders
     ts.release()
 
 

@@ -881,15 +881,16 @@ struct ts_tableset {
     {
         return {wake_tol[w], settle_ulps, wake_ulps, first_rounds, lean[w]};
     }
-    hipGraphExec_t graphs[2][2][4][2] = {};  // [FP64][counting][round order][wake_tol > 0]
-    SolveKey graph_key[2][2][4][2] = {};     // what each graph was captured with
+    static constexpr int kRoundOrders = 5;  // how a round reads the dirty flags (round_order)
+    hipGraphExec_t graphs[2][2][kRoundOrders][2] = {};  // [FP64][counting][round order][wake_tol > 0]
+    SolveKey graph_key[2][2][kRoundOrders][2] = {};     // what each graph was captured with
     // chunks of groups sweeping side by side, each on its own stream with its own graph of plane launches
     static constexpr int kMaxStreams = 8;
     hipStream_t gstream[kMaxStreams] = {};
     hipEvent_t gfork = nullptr, gjoin[kMaxStreams] = {};
-    hipGraphExec_t cgraphs[2][2][4][2][kMaxStreams] = {};
-    SolveKey cgraph_key[2][2][4][2][kMaxStreams] = {};
-    int cgraph_groups[2][2][4][2][kMaxStreams][2] = {};  // the groups [first, first + count) each swept
+    hipGraphExec_t cgraphs[2][2][kRoundOrders][2][kMaxStreams] = {};
+    SolveKey cgraph_key[2][2][kRoundOrders][2][kMaxStreams] = {};
+    int cgraph_groups[2][2][kRoundOrders][2][kMaxStreams][2] = {};  // the groups [first, first + count) each swept
     // streaming rounds, per width (FP32 / FP64 times): tile shape, the task list of every group's tiles
     // in an order where each tile follows its upwind tiles, and the queues and progress of a round
     struct StreamPlan {
@@ -1091,6 +1092,8 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
 constexpr int kWPB = TS_WPB;  // waves per block of the plane kernels
 constexpr int kSparse = 2;  // round order after kFlagWithLoads, kFlagFirst: kSparseNodes flags per wave, then the dirty nodes
 constexpr int kStream = 3;  // the streaming sweep: one persistent launch per ordering over column tiles
+constexpr int kSparseWide = 4;  // as kSparse with kWideNodes flags per wave: the late rounds, few nodes dirty
+static_assert(kSparseWide < ts_tableset::kRoundOrders, "a graph slot per round order");
 constexpr int kStreamShapes = 2;  // tile shapes per width (with_shape)
 #ifndef TS_SPARSE_NODES32
 #define TS_SPARSE_NODES32 4
@@ -1111,6 +1114,14 @@ template <typename R>
 constexpr int kSparseNodes = sizeof(R) == 4 ? TS_SPARSE_NODES32 : TS_SPARSE_NODES64;
 template <typename R>
 constexpr int kInFlight = sizeof(R) == 4 ? TS_INFLIGHT32 : TS_INFLIGHT64;
+#ifndef TS_WIDE_NODES32
+#define TS_WIDE_NODES32 8
+#endif
+#ifndef TS_WIDE_NODES64
+#define TS_WIDE_NODES64 4
+#endif
+template <typename R>
+constexpr int kWideNodes = sizeof(R) == 4 ? TS_WIDE_NODES32 : TS_WIDE_NODES64;
 
 // Cap on the blocks of a plane launch over all its groups (TOMOSTAR_PLANE_BLOCKS): waves stride over
 // the plane beyond it. Measured on the 400-field case: fewer, longer-lived waves are slower (the
@@ -1149,8 +1160,8 @@ void launch_planes(ts_tableset &t, R *T, hipStream_t st, int g0, int gc)
     for (int dir = 0; dir < 8; dir++)
         for (int p = 0; p < P; p++) {
             int off = t.plane_off[p], cnt = t.plane_off[p + 1] - off;
-            if constexpr (kOrder == kSparse) {
-                constexpr int M = kSparseNodes<R>, K = kInFlight<R>;
+            if constexpr (kOrder == kSparse || kOrder == kSparseWide) {
+                constexpr int M = kOrder == kSparse ? kSparseNodes<R> : kWideNodes<R>, K = kInFlight<R>;
                 const unsigned nb = blocks_for(cnt, kWPB * M);
                 dim3 grid((nb + 7) / 8 * 8, gc);
                 if (off32)
@@ -1327,10 +1338,12 @@ void launch_planes_lean(ts_tableset &t, R *T, hipStream_t st, bool count, int or
     if (count) {
         if (order == kFlagWithLoads) launch_planes<R, true, kFlagWithLoads, kLean>(t, T, st, g0, gc);
         else if (order == kFlagFirst) launch_planes<R, true, kFlagFirst, kLean>(t, T, st, g0, gc);
+        else if (order == kSparseWide) launch_planes<R, true, kSparseWide, kLean>(t, T, st, g0, gc);
         else launch_planes<R, true, kSparse, kLean>(t, T, st, g0, gc);
     } else {
         if (order == kFlagWithLoads) launch_planes<R, false, kFlagWithLoads, kLean>(t, T, st, g0, gc);
         else if (order == kFlagFirst) launch_planes<R, false, kFlagFirst, kLean>(t, T, st, g0, gc);
+        else if (order == kSparseWide) launch_planes<R, false, kSparseWide, kLean>(t, T, st, g0, gc);
         else launch_planes<R, false, kSparse, kLean>(t, T, st, g0, gc);
     }
 }
@@ -1366,6 +1379,7 @@ void launch_round_order(ts_tableset &t, R *T, hipStream_t st, int hold, int orde
         else launch_stream_round<R, kCount, kLean>(t, T, st, hold);
     } else if (order == kFlagWithLoads) launch_round<R, kCount, kFlagWithLoads, kLean>(t, T, st, hold);
     else if (order == kFlagFirst) launch_round<R, kCount, kFlagFirst, kLean>(t, T, st, hold);
+    else if (order == kSparseWide) launch_round<R, kCount, kSparseWide, kLean>(t, T, st, hold);
     else launch_round<R, kCount, kSparse, kLean>(t, T, st, hold);
 }
 
@@ -1401,6 +1415,12 @@ void launch_round_as(ts_tableset &t, R *T, hipStream_t st, int hold, bool count,
 // 4; 64 flags per wave from a late round on is slower in every round tried (FP64 rounds 2, 3, 4: 1.103,
 // 0.761, 0.584 s against 0.569 s; FP32 rounds 1, 2: 0.617, 0.407 s against 0.226 s) - the late rounds
 // still hold dirty nodes enough that a wave working through 64 flags' worth becomes the launch's tail.
+// - the second-order phase reads kWideNodes flags per wave from its round 5 (TOMOSTAR_SECOND_WIDE_FROM;
+//   TOMOSTAR_WIDE_FROM for the first-order phase, off by default): a round that finds few nodes dirty is
+//   bound by the waves that read the flags. Measured on the same case (results identical to the bit),
+//   four flags per wave take FP64 round 5 from 18 to 14 ms but rounds 2-4 from 96, 44, 24 ms to 99, 50,
+//   25 ms, and eight take the first-order phase's rounds 1-2 from 62, 30 ms to 69, 38 ms: wide from
+//   round 5, 0.711 s per solve; from 2, 3, 4: 0.721, 0.719, 0.712 s; never: 0.716 s.
 // - rounds before TOMOSTAR_STREAM_ROUNDS (a run from the seeds) or TOMOSTAR_STREAM_ROUNDS2 (the mixed
 //   form's second-order phase) run the streaming sweep.
 int round_order(bool all_dirty, bool one_phase, int round)
@@ -1416,7 +1436,9 @@ int round_order(bool all_dirty, bool one_phase, int round)
     int sparse = all_dirty ? first : 0;
     if (const char *e = getenv(all_dirty ? "TOMOSTAR_SECOND_SPARSE_FROM" : "TOMOSTAR_SPARSE_FROM"))
         sparse = std::max(0, atoi(e));
-    return round < first ? kFlagWithLoads : round < sparse || !one_phase ? kFlagFirst : kSparse;
+    int wide = all_dirty ? 5 : 1 << 20;
+    if (const char *e = getenv(all_dirty ? "TOMOSTAR_SECOND_WIDE_FROM" : "TOMOSTAR_WIDE_FROM")) wide = std::max(0, atoi(e));
+    return round < first ? kFlagWithLoads : round < sparse || !one_phase ? kFlagFirst : round < wide ? kSparse : kSparseWide;
 }
 
 // One round as a graph: 8 orderings x all planes x all groups, then the per-field bookkeeping. The

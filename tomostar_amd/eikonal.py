@@ -171,7 +171,8 @@ class TableSet:
 
     def round_log(self):
         """[(width, order, seconds)] per round of the last fast solve: width 32 or 64 (the times swept),
-        order 0 (flag with every load), 1 (flag first), 2 (a few flags per wave), 3 (streaming tiles)."""
+        order 0 (flag with every load), 1 (flag first), 2 (a few flags per wave), 3 (streaming tiles),
+        4 (more flags per wave)."""
         L = _lib.load()
         n = L.ts_tables_round_log(self._h, 0, None, None)
         check(min(n, 0))
