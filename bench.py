@@ -48,7 +48,8 @@ def log(msg: str) -> None:
 def host_cpu() -> dict:
     """The host's CPUs: nproc, the CPUs this process may run on (sched_getaffinity), the threads the
     CPU baselines use, and the CPU model. The threads are every CPU of the affinity set, capped by
-    OMP_NUM_THREADS when it is set: the GPU box gives one GPU's jobs a share of its cores through it."""
+    OMP_NUM_THREADS when it is set: a host shared by several GPUs' jobs gives each its share of the
+    cores through it."""
     avail = len(os.sched_getaffinity(0))
     share = os.environ.get("OMP_NUM_THREADS", "")
     threads = min(avail, int(share)) if share.isdigit() and int(share) > 0 else avail

@@ -333,7 +333,7 @@ def test_suggest_batch_follows_the_reference_rule(device):
     assert L.ts_eik_suggest_batch(device.handle, 4_080_501) == 24  # 768 MiB of FP64 host buffers: 24 fields of 201 x 201 x 101
 
 
-def test_host_seed_and_metric_rows_on_the_device_box(oracle):
+def test_host_seed_and_metric_rows_beside_the_device(oracle):
     """GridMetric and PointSource against the oracle to the last bit, run where the GPU tests run."""
     import test_abi
 
