@@ -272,6 +272,32 @@ def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
     ts.release()
 
 
+@pytest.mark.parametrize("nst", [70, 40, 50])
+def test_narrow_group_four_nodes_per_wave(device, monkeypatch, nst):
+    """A last group of 1-16 fields (140 fields: 12 over two groups; 80: 16 over one) updates four plane
+    nodes per wave in the first-order phase, 16 lanes each; 100 fields leave 36, a full-width group. The
+    times of the mixed form are the same to the bit as with one node per wave (TOMOSTAR_NARROW=0), and
+    so are the counted updates."""
+    from tomostar_amd import eikonal as E
+    from tomostar_amd import workloads as W
+
+    wl = _checker_case(11, nx=37, ny=33, nz=19, nst=nst)
+    seeds, mof = W.station_seed_sets(wl, E.point_source)
+    out = {}
+    for narrow in ("0", "1"):
+        monkeypatch.setenv("TOMOSTAR_NARROW", narrow)
+        print(f"fields {2 * nst} narrow {narrow}", flush=True)
+        ts = E.TableSet(device, wl.grid, [wl.slowness_p, wl.slowness_s], seeds, mof)
+        rc, why = ts.solve("fast_fp32")
+        assert rc == 0, why
+        st = ts.stats()
+        out[narrow] = (ts.read(), st["updates_fp32"], st["updates_fp64"])
+        ts.release()
+    assert np.isfinite(out["1"][0]).any()
+    assert np.array_equal(out["0"][0], out["1"][0], equal_nan=True)
+    assert out["0"][1:] == out["1"][1:]
+
+
 def test_first_order_wake_threshold(device, oracle, monkeypatch):
     """The mixed form's first-order phase wakes a node's readers only on drops above
     TOMOSTAR_FP32_WAKE_ULPS ulps of its time (16 by default): the times stay within the bar of Relax

@@ -239,6 +239,7 @@ struct FastArgs {
     float settle_ulps;            // FP32 times: a node moved when it dropped by more than this many ulps
     float wake_ulps;              // FP32 times: a drop wakes the node's readers when larger than this many ulps
     int nx, ny, nz, N;
+    int narrow;                   // the group of at most 16 fields whose waves update four nodes (-1: none)
 };
 
 // A load at a uniform base plus a 32-bit byte offset (one address register per lane).
@@ -570,6 +571,96 @@ constexpr bool kFirstOnly = sizeof(R) == 4;
 
 namespace ts {
 
+// A group of at most 16 fields (a table set's last group, when its field count is 1-16 over a multiple
+// of 64): a wave updates four plane nodes at once, 16 lanes each (lane = 16 x quarter + field), so the
+// group takes a quarter of the waves of a full one. Per lane, the loads, the arithmetic, the tests and
+// the flags are those of the one-node-per-wave kernels (finish_node), and no node of a plane reads
+// another node of the same plane, so the times are the same to the bit. Plane nodes a0 + quarter,
+// a0 + stride + quarter, ... (a0 and stride wave-uniform, multiples of 4). Times and slowness by 32-bit
+// byte offsets from uniform bases (the caller checks they fit), one address register per load.
+template <typename R, bool kCount, bool kLean, bool kSecond>
+__device__ __forceinline__ void narrow_nodes(const FastArgs<R> &A, int dir, int off, int cnt, int g, uint64_t act,
+                                             uint64_t ph, int a0, int stride)
+{
+    const int lane = threadIdx.x & 63, qd = lane >> 4, fl = lane & 15;
+    const int nx = A.nx, ny = A.ny, nz = A.nz, nxy = nx * ny;
+    const size_t gn = (size_t)g * A.N;
+    const R *Tg = A.T + gn * kLanes;
+    const uint8_t *dg = A.dirty + gn;
+    const int ll = load_lane(act, fl);  // the group's fields are lanes 0-15
+    const int mdl = A.model[g * kLanes + ll];
+    const int wake_off = wake_offset(fl, nx, nxy);
+    uint64_t mv = 0;
+    for (; a0 < cnt; a0 += stride) {
+        const int a = a0 + qd;
+        NodeLoad<R> L;
+        L.fl = 0;
+        L.node = L.i = L.j = L.k = 0;
+        if (a < cnt) {
+            L.node = plane_node(A, dir, A.planes[off + a], L.i, L.j, L.k);
+            L.fl = dg[L.node];
+        }
+        bool dropped = false, moved = false;
+        if (L.fl) {
+            const int node = L.node;
+            auto T = [&](int q) -> R { return at_off(Tg, ((uint32_t)q * kLanes + ll) * (uint32_t)sizeof(R)); };
+            L.fixed = A.seedmask[gn + node];
+            L.s = at_off(A.slow, ((uint32_t)mdl * (uint32_t)A.N + node) * (uint32_t)sizeof(R));
+            L.cur = T(node);
+            L.wlon0 = A.wlon[2 * (L.k * ny + L.j)];
+            L.wlon1 = A.wlon[2 * (L.k * ny + L.j) + 1];
+            L.wlat0 = A.wlat[2 * L.k];
+            L.wlat1 = A.wlat[2 * L.k + 1];
+            const int pos[3] = {L.i, L.j, L.k}, n[3] = {nx, ny, nz}, st[3] = {1, nx, nxy};
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) {
+                L.mn[ax] = pos[ax] > 0 ? T(node - st[ax]) : (R)INFINITY;
+                L.pl[ax] = pos[ax] + 1 < n[ax] ? T(node + st[ax]) : (R)INFINITY;
+                L.m2[ax] = kSecond && pos[ax] > 1 ? T(node - 2 * st[ax]) : (R)INFINITY;
+                L.p2[ax] = kSecond && pos[ax] + 2 < n[ax] ? T(node + 2 * st[ax]) : (R)INFINITY;
+            }
+            if (fl == 0) A.dirty[gn + node] = 0;
+            if (kCount && fl == 0) atomicAdd(&A.evals[blockIdx.x & 1023], (unsigned long long)__popcll(act & ~L.fixed));
+            bool on;
+            const R next = node_update<R, kLean, kSecond>(A, L, act, ph, fl, on);
+            if (on && next < L.cur) {
+                A.T[(gn + node) * kLanes + fl] = next;
+                drop_tests(A, L.cur, next, dropped, moved);
+            }
+        }
+        // a quarter whose node dropped for some field wakes the node's readers (its lanes 0-11)
+        if ((__ballot(dropped) >> (16 * qd)) & 0xFFFFull) {
+            const int pos[3] = {L.i, L.j, L.k}, n[3] = {nx, ny, nz};
+            uint32_t in = 0;
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++)
+                in |= (uint32_t)((pos[ax] >= 1) | (pos[ax] >= 2) << 1 | (pos[ax] + 1 < n[ax]) << 2 |
+                                 (pos[ax] + 2 < n[ax]) << 3) << (4 * ax);
+            if (!kSecond) in &= 0x555u;
+            if ((in >> fl) & 1) A.dirty[gn + L.node + wake_off] = 1;
+        }
+        const uint64_t m = __ballot(moved);
+        mv |= (m | m >> 16 | m >> 32 | m >> 48) & 0xFFFFull;
+    }
+    mark_changed(&A.changed[g], mv, lane);
+}
+
+// The narrow group's part of a plane launch (g == A.narrow): its waves take four nodes each, the
+// kernel's XCD slices of the plane in steps of four.
+template <typename R, bool kCount, bool kLean, int WPB>
+__device__ __forceinline__ void narrow_plane(const FastArgs<R> &A, int dir, int off, int cnt, int g, uint64_t act)
+{
+    const int per_xcd = (gridDim.x + 7) / 8;
+    const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    const int a0 = (lb * WPB + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * 4;
+    const int stride = 8 * per_xcd * WPB * 4;
+    const uint64_t ph = A.phase1[g];
+    if constexpr (kFirstOnly<R>) narrow_nodes<R, kCount, kLean, false>(A, dir, off, cnt, g, act, ph, a0, stride);
+    else if (kSecondOnly<R, kLean> || (ph & act) != 0)
+        narrow_nodes<R, kCount, kLean, true>(A, dir, off, cnt, g, act, ph, a0, stride);
+    else narrow_nodes<R, kCount, kLean, false>(A, dir, off, cnt, g, act, ph, a0, stride);
+}
+
 // The nodes a wave of eik_fast_plane updates: plane nodes a, a + stride, ... (kSecond: some field of
 // the group in its second-order phase, so the loads two nodes away are issued).
 template <typename R, bool kCount, bool kOff32, int kFlag, bool kLean, bool kSecond>
@@ -594,13 +685,19 @@ __device__ __forceinline__ void plane_nodes(const FastArgs<R> &A, int dir, int o
 // flag read first (rounds where most nodes are clean) or with every other load (where most are not).
 // The all-loads variant is held to 8 waves per SIMD (its FP64 forms otherwise take 101 scalar registers
 // and 7 waves; held, 78 and no spills): the round it runs, where every node is dirty, is latency-bound.
-template <typename R, int WPB, bool kCount, bool kOff32, int kFlag, bool kLean>
+template <typename R, int WPB, bool kCount, bool kOff32, int kFlag, bool kLean, bool kNarrow = false>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(kFlag == kFlagWithLoads ? 8 : 1))) void eik_fast_plane(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
     const int g = g0 + blockIdx.y;
     const uint64_t act = A.active[g];
     if (act == 0) return;
+    if constexpr (kNarrow && kOff32) {
+        if (g == A.narrow) {
+            narrow_plane<R, kCount, kLean, WPB>(A, dir, off, cnt, g, act);
+            return;
+        }
+    }
     // Workgroups go to the eight XCDs in turn (block b to XCD b mod 8); block b takes the b/8-th slice
     // of XCD (b mod 8)'s contiguous share of the plane, so the nodes one XCD updates are neighbours and
     // the rows their stencils share meet in that XCD's L2.
@@ -653,13 +750,19 @@ __device__ __forceinline__ void sparse_nodes(const FastArgs<R> &A, int dir, int 
 // nodes at once (lanes 0 .. M-1) and then updates its dirty ones in plane order, so a clean stretch of
 // the plane costs one load per M nodes and M times fewer waves are launched. FP32: held to 7 waves per
 // SIMD (72 vector registers instead of 76 and 6 waves, no spills; 8 waves would spill).
-template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean, int K>
+template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean, int K, bool kNarrow = false>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? TS_SPARSE_WAVES32 : TS_SPARSE_WAVES64))) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
     const int g = g0 + blockIdx.y;
     const uint64_t act = A.active[g];
     if (act == 0) return;
+    if constexpr (kNarrow && kOff32) {
+        if (g == A.narrow) {
+            narrow_plane<R, kCount, kLean, WPB>(A, dir, off, cnt, g, act);
+            return;
+        }
+    }
     const int per_xcd = (gridDim.x + 7) / 8;
     const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     const int a0 = (lb * WPB + (threadIdx.x >> 6)) * M;
@@ -864,6 +967,7 @@ struct ts_tableset {
                                           // cap; TOMOSTAR_FP32_MAX_ROUNDS)
     // launch-shape knobs, read once when the table set is created (graphs captured with them stay valid)
     int knob_group_chunk = 0, knob_plane_blocks = 0;
+    bool narrow = true;   // a last group of 1-16 fields updates four nodes per wave (TOMOSTAR_NARROW=0: not)
     // what a captured graph's kernel arguments hold of the running solve: its tolerances, thresholds
     // and arithmetic
     struct SolveKey {
@@ -1083,6 +1187,8 @@ FastArgs<R> fast_args(ts_tableset &t, R *T)
     A.settle_ulps = t.settle_ulps;
     A.wake_ulps = t.wake_ulps;
     A.nx = t.grid.nx; A.ny = t.grid.ny; A.nz = t.grid.nz; A.N = t.N;
+    const int rest = t.nfields % kLanes;
+    A.narrow = t.narrow && rest >= 1 && rest <= 16 ? t.nfields / kLanes : -1;
     return A;
 }
 
@@ -1146,17 +1252,21 @@ void read_launch_knobs(ts_tableset &t)
     if (const char *e = getenv("TOMOSTAR_STREAM_PUBLISH")) t.stream_publish = std::clamp(atoi(e), 1, 64);
     if (const char *e = getenv("TOMOSTAR_GROUP_CHUNK")) t.knob_group_chunk = std::max(1, atoi(e));
     if (const char *e = getenv("TOMOSTAR_PLANE_BLOCKS")) t.knob_plane_blocks = std::max(1, atoi(e));
+    if (const char *e = getenv("TOMOSTAR_NARROW")) t.narrow = atoi(e) != 0;
 }
 
-// The plane launches of groups [g0, g0 + gc): all eight orderings, every plane.
-template <typename R, bool kCount, int kOrder, bool kLean>
-void launch_planes(ts_tableset &t, R *T, hipStream_t st, int g0, int gc)
+// The plane launches of groups [g0, g0 + gc): all eight orderings, every plane. On FP32 times, a chunk
+// holding the narrow group (FastArgs::narrow) launches the kernels that carry its four-nodes-per-wave
+// body; the others launch kernels without it (the body takes vector registers the full groups' waves
+// would then lack). Measured on 400 fields (six groups and one of 16) of 201 x 201 x 101: the first-order
+// rounds 79, 58, 30 ms against 81, 62, 30 ms, the solve 0.705 s against 0.712 s, the times the same to
+// the bit. In the FP64 kernels the body takes the sparse kernel from 58 to 72 vector registers (8 waves
+// per SIMD to 7) and gives the all-loads kernel scratch: the second-order rounds 208, 162, 100 ms
+// against 187, 159, 96 ms, so the FP64 times keep one node per wave.
+template <typename R, bool kCount, int kOrder, bool kLean, bool kNarrow>
+void launch_planes_n(ts_tableset &t, const FastArgs<R> &A, R *T, hipStream_t st, int g0, int gc, bool off32)
 {
-    FastArgs<R> A = fast_args<R>(t, T);
     const int P = t.grid.nx + t.grid.ny + t.grid.nz - 2;
-    // 32-bit byte offsets when a group's times and all slowness models fit in 4 GB
-    const bool off32 = (uint64_t)t.N * kLanes * sizeof(R) < (1ull << 32) &&
-                       (uint64_t)t.nmodels * t.N * sizeof(R) < (1ull << 32);
     for (int dir = 0; dir < 8; dir++)
         for (int p = 0; p < P; p++) {
             int off = t.plane_off[p], cnt = t.plane_off[p + 1] - off;
@@ -1165,17 +1275,35 @@ void launch_planes(ts_tableset &t, R *T, hipStream_t st, int g0, int gc)
                 const unsigned nb = blocks_for(cnt, kWPB * M);
                 dim3 grid((nb + 7) / 8 * 8, gc);
                 if (off32)
-                    eik_fast_plane_sparse<R, kWPB, M, kCount, true, kLean, K><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                    eik_fast_plane_sparse<R, kWPB, M, kCount, true, kLean, K, kNarrow><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
                 else
                     eik_fast_plane_sparse<R, kWPB, M, kCount, false, kLean, K><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
             } else {
                 // a multiple of 8 blocks: each XCD gets the same number of slices
                 const unsigned nb = std::min<unsigned>(blocks_for(cnt, kWPB), std::max(1, plane_blocks(t) / gc));
                 dim3 grid((nb + 7) / 8 * 8, gc);
-                if (off32) eik_fast_plane<R, kWPB, kCount, true, kOrder, kLean><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
+                if (off32) eik_fast_plane<R, kWPB, kCount, true, kOrder, kLean, kNarrow><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
                 else eik_fast_plane<R, kWPB, kCount, false, kOrder, kLean><<<grid, 64 * kWPB, 0, st>>>(A, dir, off, cnt, g0);
             }
         }
+}
+
+template <typename R, bool kCount, int kOrder, bool kLean>
+void launch_planes(ts_tableset &t, R *T, hipStream_t st, int g0, int gc)
+{
+    FastArgs<R> A = fast_args<R>(t, T);
+    // 32-bit byte offsets when a group's times and all slowness models fit in 4 GB
+    const bool off32 = (uint64_t)t.N * kLanes * sizeof(R) < (1ull << 32) &&
+                       (uint64_t)t.nmodels * t.N * sizeof(R) < (1ull << 32);
+    // FP32 times only: in the FP64 kernels the body costs the full groups of its chunk more than it saves
+    if constexpr (sizeof(R) == 4) {
+        if (off32 && A.narrow >= g0 && A.narrow < g0 + gc) {
+            launch_planes_n<R, kCount, kOrder, kLean, true>(t, A, T, st, g0, gc, off32);
+            return;
+        }
+    }
+    A.narrow = -1;
+    launch_planes_n<R, kCount, kOrder, kLean, false>(t, A, T, st, g0, gc, off32);
 }
 
 // The per-field bookkeeping that ends a round.
