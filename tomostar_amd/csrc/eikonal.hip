@@ -616,8 +616,17 @@ __device__ __forceinline__ void narrow_nodes(const FastArgs<R> &A, int dir, int 
             for (int ax = 0; ax < 3; ax++) {
                 L.mn[ax] = pos[ax] > 0 ? T(node - st[ax]) : (R)INFINITY;
                 L.pl[ax] = pos[ax] + 1 < n[ax] ? T(node + st[ax]) : (R)INFINITY;
-                L.m2[ax] = kSecond && pos[ax] > 1 ? T(node - 2 * st[ax]) : (R)INFINITY;
-                L.p2[ax] = kSecond && pos[ax] + 2 < n[ax] ? T(node + 2 * st[ax]) : (R)INFINITY;
+            }
+            // the second neighbour beyond the smaller first one only (node_update reads no other): three
+            // values live rather than six
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) {
+                R t2 = (R)INFINITY;
+                if constexpr (kSecond) {
+                    if (L.mn[ax] <= L.pl[ax]) { if (pos[ax] > 1) t2 = T(node - 2 * st[ax]); }
+                    else if (pos[ax] + 2 < n[ax]) t2 = T(node + 2 * st[ax]);
+                }
+                L.m2[ax] = L.p2[ax] = t2;
             }
             if (fl == 0) A.dirty[gn + node] = 0;
             if (kCount && fl == 0) atomicAdd(&A.evals[blockIdx.x & 1023], (unsigned long long)__popcll(act & ~L.fixed));
@@ -751,7 +760,7 @@ __device__ __forceinline__ void sparse_nodes(const FastArgs<R> &A, int dir, int 
 // the plane costs one load per M nodes and M times fewer waves are launched. FP32: held to 7 waves per
 // SIMD (72 vector registers instead of 76 and 6 waves, no spills; 8 waves would spill).
 template <typename R, int WPB, int M, bool kCount, bool kOff32, bool kLean, int K, bool kNarrow = false>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? TS_SPARSE_WAVES32 : TS_SPARSE_WAVES64))) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? TS_SPARSE_WAVES32 : kNarrow ? 8 : TS_SPARSE_WAVES64))) void eik_fast_plane_sparse(FastArgs<R> A, int dir, int off, int cnt, int g0)
 {
     const int lane = threadIdx.x & 63;
     const int g = g0 + blockIdx.y;
@@ -1255,14 +1264,15 @@ void read_launch_knobs(ts_tableset &t)
     if (const char *e = getenv("TOMOSTAR_NARROW")) t.narrow = atoi(e) != 0;
 }
 
-// The plane launches of groups [g0, g0 + gc): all eight orderings, every plane. On FP32 times, a chunk
-// holding the narrow group (FastArgs::narrow) launches the kernels that carry its four-nodes-per-wave
-// body; the others launch kernels without it (the body takes vector registers the full groups' waves
-// would then lack). Measured on 400 fields (six groups and one of 16) of 201 x 201 x 101: the first-order
-// rounds 79, 58, 30 ms against 81, 62, 30 ms, the solve 0.705 s against 0.712 s, the times the same to
-// the bit. In the FP64 kernels the body takes the sparse kernel from 58 to 72 vector registers (8 waves
-// per SIMD to 7) and gives the all-loads kernel scratch: the second-order rounds 208, 162, 100 ms
-// against 187, 159, 96 ms, so the FP64 times keep one node per wave.
+// The plane launches of groups [g0, g0 + gc): all eight orderings, every plane. A chunk holding the
+// narrow group (FastArgs::narrow) launches the kernels that carry its four-nodes-per-wave body, on FP32
+// times and in the FP64 rounds that read two flags per wave; the others launch kernels without it (the
+// body takes vector registers the full groups' waves would then lack: the FP64 sparse kernel carrying it
+// is held to 8 waves per SIMD, 64 registers). Measured on 400 fields (six groups and one of 16) of
+// 201 x 201 x 101, the times the same to the bit: first-order rounds 79, 58, 30 ms against 81, 61, 30 ms,
+// second-order rounds 1-2 151, 92 ms against 158, 96 ms, the solve 0.695 s against 0.710 s. In the FP64
+// all-loads round (186 ms against 193 ms) and the late rounds (four flags per wave: 14 against 17 ms)
+// the body costs the chunk's full groups more than it saves.
 template <typename R, bool kCount, int kOrder, bool kLean, bool kNarrow>
 void launch_planes_n(ts_tableset &t, const FastArgs<R> &A, R *T, hipStream_t st, int g0, int gc, bool off32)
 {
@@ -1295,8 +1305,7 @@ void launch_planes(ts_tableset &t, R *T, hipStream_t st, int g0, int gc)
     // 32-bit byte offsets when a group's times and all slowness models fit in 4 GB
     const bool off32 = (uint64_t)t.N * kLanes * sizeof(R) < (1ull << 32) &&
                        (uint64_t)t.nmodels * t.N * sizeof(R) < (1ull << 32);
-    // FP32 times only: in the FP64 kernels the body costs the full groups of its chunk more than it saves
-    if constexpr (sizeof(R) == 4) {
+    if constexpr (sizeof(R) == 4 || kOrder == kSparse) {  // where it pays (above)
         if (off32 && A.narrow >= g0 && A.narrow < g0 + gc) {
             launch_planes_n<R, kCount, kOrder, kLean, true>(t, A, T, st, g0, gc, off32);
             return;
