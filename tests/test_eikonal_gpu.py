@@ -275,8 +275,8 @@ def test_mixed_schedule_leaves_the_times_unchanged(device, monkeypatch):
 @pytest.mark.parametrize("nst", [70, 40, 50])
 def test_narrow_group_four_nodes_per_wave(device, monkeypatch, nst):
     """A last group of 1-16 fields (140 fields: 12 over two groups; 80: 16 over one) updates four plane
-    nodes per wave, 16 lanes each, in the first-order phase and the second-order phase's two-flag
-    rounds; 100 fields leave 36, a full-width group. The
+    nodes per wave, 16 lanes each, in the first-order phase and the second-order phase's rounds
+    before the late four-flag ones; 100 fields leave 36, a full-width group. The
     times of the mixed form are the same to the bit as with one node per wave (TOMOSTAR_NARROW=0), and
     so are the counted updates."""
     from tomostar_amd import eikonal as E

@@ -1270,9 +1270,11 @@ void read_launch_knobs(ts_tableset &t)
 // body takes vector registers the full groups' waves would then lack: the FP64 sparse kernel carrying it
 // is held to 8 waves per SIMD, 64 registers). Measured on 400 fields (six groups and one of 16) of
 // 201 x 201 x 101, the times the same to the bit: first-order rounds 79, 58, 30 ms against 81, 61, 30 ms,
-// second-order rounds 1-2 151, 92 ms against 158, 96 ms, the solve 0.695 s against 0.710 s. In the FP64
-// all-loads round (186 ms against 193 ms) and the late rounds (four flags per wave: 14 against 17 ms)
-// the body costs the chunk's full groups more than it saves.
+// second-order rounds 1-2 151, 92 ms against 158, 96 ms. In the FP64 all-loads round the body costs the
+// chunk's full groups more than it saves (193 ms against 186 ms), but that chunk's launches reading two
+// flags per wave with it take the round to 180 ms (the full groups run that round as fast either way):
+// the solve 0.689 s against 0.710 s. The late rounds (four flags per wave) keep one node per wave (with
+// the body: 17 ms against 14 ms).
 template <typename R, bool kCount, int kOrder, bool kLean, bool kNarrow>
 void launch_planes_n(ts_tableset &t, const FastArgs<R> &A, R *T, hipStream_t st, int g0, int gc, bool off32)
 {
@@ -1308,6 +1310,12 @@ void launch_planes(ts_tableset &t, R *T, hipStream_t st, int g0, int gc)
     if constexpr (sizeof(R) == 4 || kOrder == kSparse) {  // where it pays (above)
         if (off32 && A.narrow >= g0 && A.narrow < g0 + gc) {
             launch_planes_n<R, kCount, kOrder, kLean, true>(t, A, T, st, g0, gc, off32);
+            return;
+        }
+    } else if constexpr (kOrder == kFlagWithLoads) {
+        // the all-loads round: that chunk reads two flags per wave instead, with the body
+        if (off32 && A.narrow >= g0 && A.narrow < g0 + gc) {
+            launch_planes_n<R, kCount, kSparse, kLean, true>(t, A, T, st, g0, gc, off32);
             return;
         }
     }
