@@ -36,7 +36,7 @@ BYTES_PER_UPDATE_FP64 = 20.0
 # counters read for an elementwise copy of 1 GiB (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000). The runs and
 # the figures are summarised in profiles/eik_fast_plane_batched_400x201x201x101.md.
 TRAFFIC_BYTES_PER_LAUNCH: dict = {
-    "eik_fast_plane<double>": 7.944e7,
+    "eik_fast_plane<double>": 8.113e7,
     "eik_fast_plane<float>": 3.728e7,
 }
 
